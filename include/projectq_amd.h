@@ -1,0 +1,149 @@
+/*
+ * projectq_amd.h -- C ABI of the MI355X state-vector simulator.
+ *
+ * One opaque handle per simulator.  Every entry point mirrors one method of the pybind11 class
+ * `projectq.backends._sim._cppsim.Simulator` (projectq/backends/_sim/_cppsim.cpp:43-67), whose
+ * implementation lives in projectq/backends/_sim/_cppkernels/simulator.hpp.  The Python binding
+ * (projectq_amd/_cppsim.py) keeps that class's method names and argument conventions, so
+ * `projectq.backends.Simulator()._simulator = projectq_amd.Simulator(seed)` is a drop-in.
+ *
+ * Conventions shared by all entry points
+ *  - Qubit ids are uint32.  A gate matrix of k target qubits is 2^k x 2^k complex, passed
+ *    row-major as interleaved (re, im) doubles: m[2*(r*2^k + c)] = Re M[r][c].  Bit l of the
+ *    row/column index belongs to ids[l] (simulator.hpp:487-527 with fusion.hpp:65-109).
+ *  - Register layouts (emulate_math) are passed as all qubit ids concatenated plus one size per
+ *    register; register r's qubit j is bit j of the register value (little endian).
+ *  - Pauli terms (expectation value, qubit operator, time evolution) are passed as
+ *    concatenated (local index, op) pairs plus offsets[n_terms + 1]; op is 'X', 'Y' or 'Z' and
+ *    local index l refers to ids[l] (simulator.hpp:538-550).
+ *  - Bits / booleans are uint8 (0 or 1).
+ *  - Every function returns QS_OK or an error class; qs_last_error() holds the message of the
+ *    last failure on the calling thread.  The classes map to the C++ exception types the
+ *    reference throws: runtime_error -> QS_ERR_RUNTIME, length_error/invalid_argument ->
+ *    QS_ERR_VALUE, bad_alloc -> QS_ERR_MEMORY.
+ *
+ * Thread safety: a handle must not be used from two threads at once (same as the reference,
+ * which is called under the GIL).  Different handles are independent: unlike the reference's
+ * process-wide static scratch buffers (simulator.hpp:574-578), every handle owns its buffers.
+ */
+#ifndef PROJECTQ_AMD_H_
+#define PROJECTQ_AMD_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct qs_simulator qs_simulator;
+
+/* status codes */
+#define QS_OK 0
+#define QS_ERR_RUNTIME 1  /* std::runtime_error   -> RuntimeError */
+#define QS_ERR_VALUE 2    /* std::length_error / std::invalid_argument -> ValueError */
+#define QS_ERR_MEMORY 3   /* std::bad_alloc / hipErrorOutOfMemory -> MemoryError */
+#define QS_ERR_DEVICE 4   /* any other HIP failure -> RuntimeError */
+#define QS_ERR_CALLBACK 5 /* a user callback (emulate_math) reported failure */
+
+/* precision of the resident state */
+#define QS_C128 0 /* complex<double>, the reference's calc_type (simulator.hpp:39-40) */
+#define QS_C64 1  /* complex<float> state; norms, probabilities and sums stay FP64 */
+
+/* how apply_controlled_gate queues gates */
+#define QS_FUSION_REFERENCE 0 /* greedy min-4/max-5 rule of simulator.hpp:204-222, run() drains */
+#define QS_FUSION_WINDOW 1    /* queue every gate until run(); the planner partitions the window */
+
+/* planner used for QS_FUSION_WINDOW */
+#define QS_PLANNER_GREEDY 0   /* greedy max-5 blocks (fusion.hpp restated over the whole window) */
+#define QS_PLANNER_LEARNED 1  /* MLP cost model chooses block boundaries; greedy is its fallback */
+
+/* emulate_math opcodes (qs_emulate_math_op).  The first three are the reference's native fast
+ * paths (simulator.hpp:271-290); the rest are the integer functions of projectq/libs/math/_gates.py,
+ * which the reference evaluates through a Python callback. */
+#define QS_MATH_ADD_CONSTANT 0            /* x += a                      (_gates.py:20-62)  */
+#define QS_MATH_ADD_CONSTANT_MOD_N 1      /* x = (x + a) % N             (_gates.py:82-133) */
+#define QS_MATH_MULTIPLY_BY_CONSTANT_MOD_N 2 /* x = (x * a) % N          (_gates.py:164-213) */
+#define QS_MATH_ADD_QUANTUM 3             /* (a, b[, c]): b += a, wrap + flip carry (_gates.py:251-265) */
+#define QS_MATH_SUBTRACT_QUANTUM 4        /* (a, b): b -= a              (_gates.py:298-344) */
+#define QS_MATH_INVERSE_ADD_QUANTUM 5     /* (a, b[, c]): flip c, b -= a (_gates.py:268-296) */
+#define QS_MATH_COMPARATOR 6              /* (a, b, c): c ^= (b < a)     (_gates.py:347-400) */
+#define QS_MATH_MULTIPLY_QUANTUM 7        /* (a, b, c): c += a*b         (_gates.py:488-538) */
+#define QS_MATH_INVERSE_MULTIPLY_QUANTUM 8 /* (a, b, c): c -= a*b        (_gates.py:541-570) */
+#define QS_MATH_DIVIDE_QUANTUM 9          /* (dividend, remainder, divisor) (_gates.py:403-464) */
+#define QS_MATH_INVERSE_DIVIDE_QUANTUM 10 /* (remainder, quotient, divisor) (_gates.py:467-485) */
+
+/* callback for the generic emulate_math: rewrite values[0..n_regs) in place; return 0 on
+ * success, non-zero to abort the call with QS_ERR_CALLBACK. */
+typedef int (*qs_math_fn)(void *user, int32_t *values, size_t n_regs);
+
+const char *qs_last_error(void);
+int qs_abi_version(void);
+
+/* Simulator(seed) -- simulator.hpp:48-53.  capacity_hint_qubits reserves HBM for that many
+ * qubits up front (0 = grow on demand). */
+int qs_create(uint32_t seed, int precision, int fusion_mode, int planner,
+              uint32_t capacity_hint_qubits, qs_simulator **out);
+void qs_destroy(qs_simulator *sim);
+
+/* simulator.hpp:55-74 */
+int qs_allocate_qubit(qs_simulator *sim, uint32_t id);
+/* simulator.hpp:194-202 (is_classical check, then collapse_vector with shrink) */
+int qs_deallocate_qubit(qs_simulator *sim, uint32_t id);
+/* simulator.hpp:76-91 */
+int qs_get_classical_value(qs_simulator *sim, uint32_t id, double tol, int *value);
+/* simulator.hpp:93-108 */
+int qs_is_classical(qs_simulator *sim, uint32_t id, double tol, int *value);
+/* simulator.hpp:145-192 */
+int qs_measure_qubits(qs_simulator *sim, const uint32_t *ids, size_t n_ids, uint8_t *results);
+/* simulator.hpp:204-222 (+ fusion.hpp:56-63) */
+int qs_apply_controlled_gate(qs_simulator *sim, const double *matrix, const uint32_t *ids,
+                             size_t n_ids, const uint32_t *ctrl, size_t n_ctrl);
+/* simulator.hpp:224-269 with a host callback; the callback sees each distinct tuple of
+ * register values once. */
+int qs_emulate_math(qs_simulator *sim, qs_math_fn fn, void *user, const uint32_t *reg_qubits,
+                    const uint32_t *reg_sizes, size_t n_regs, const uint32_t *ctrl,
+                    size_t n_ctrl);
+/* simulator.hpp:224-290: integer functions evaluated natively (opcodes above); params holds
+ * a then N for the constant forms and is ignored otherwise. */
+int qs_emulate_math_op(qs_simulator *sim, int opcode, const int64_t *params, size_t n_params,
+                       const uint32_t *reg_qubits, const uint32_t *reg_sizes, size_t n_regs,
+                       const uint32_t *ctrl, size_t n_ctrl);
+/* simulator.hpp:292-322 -> sum_t coeff_t * Re<psi|P_t|psi> */
+int qs_get_expectation_value(qs_simulator *sim, const uint32_t *term_idx, const uint8_t *term_ops,
+                             const size_t *term_offsets, size_t n_terms, const double *coeffs,
+                             const uint32_t *ids, size_t n_ids, double *result);
+/* simulator.hpp:324-350; coeffs are complex (interleaved) */
+int qs_apply_qubit_operator(qs_simulator *sim, const uint32_t *term_idx, const uint8_t *term_ops,
+                            const size_t *term_offsets, size_t n_terms, const double *coeffs,
+                            const uint32_t *ids, size_t n_ids);
+/* simulator.hpp:386-438; coeffs are real */
+int qs_emulate_time_evolution(qs_simulator *sim, const uint32_t *term_idx,
+                              const uint8_t *term_ops, const size_t *term_offsets,
+                              size_t n_terms, const double *coeffs, double time,
+                              const uint32_t *ids, size_t n_ids, const uint32_t *ctrl,
+                              size_t n_ctrl);
+/* simulator.hpp:352-368 */
+int qs_get_probability(qs_simulator *sim, const uint8_t *bits, const uint32_t *ids, size_t n_ids,
+                       double *result);
+/* simulator.hpp:370-384; result = (re, im) */
+int qs_get_amplitude(qs_simulator *sim, const uint8_t *bits, const uint32_t *ids, size_t n_ids,
+                     double *result);
+/* simulator.hpp:440-454; wf = 2^n interleaved complex */
+int qs_set_wavefunction(qs_simulator *sim, const double *wf, size_t n_amps,
+                        const uint32_t *ordering, size_t n_ids);
+/* simulator.hpp:456-485 */
+int qs_collapse_wavefunction(qs_simulator *sim, const uint32_t *ids, size_t n_ids,
+                             const uint8_t *values, size_t n_values);
+/* simulator.hpp:487-527 */
+int qs_run(qs_simulator *sim);
+/* simulator.hpp:529-532, split in two calls: sizes, then copies into caller-owned buffers.
+ * qs_cheat_sizes runs pending gates; state is 2^n interleaved complex128. */
+int qs_cheat_sizes(qs_simulator *sim, size_t *n_qubits, size_t *n_amps);
+int qs_cheat(qs_simulator *sim, uint32_t *ids, uint32_t *positions, double *state);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PROJECTQ_AMD_H_ */

@@ -1,0 +1,453 @@
+"""ctypes binding over the simulator C ABI (include/projectq_amd.h).
+
+`CAbiSimulator` reproduces the method surface, argument conventions and exception types of the
+pybind11 class `projectq.backends._sim._cppsim.Simulator` (projectq/backends/_sim/_cppsim.cpp:43-67):
+lists in, lists/dicts/complex out, `RuntimeError` / `ValueError` / `MemoryError` raised where the
+reference's C++ throws `runtime_error` / `length_error`+`invalid_argument` / `bad_alloc`.
+
+The class is bound to one shared library at construction.  `projectq_amd.Simulator` binds it to
+the HIP library; nothing in the package binds anything else.
+"""
+
+import ctypes
+import math
+
+import numpy as np
+
+from . import _mathfuncs
+
+QS_OK = 0
+QS_ERR_RUNTIME = 1
+QS_ERR_VALUE = 2
+QS_ERR_MEMORY = 3
+QS_ERR_DEVICE = 4
+QS_ERR_CALLBACK = 5
+
+PRECISIONS = {"c128": 0, "c64": 1}
+FUSION_MODES = {"reference": 0, "window": 1}
+PLANNERS = {"greedy": 0, "learned": 1}
+
+_u32p = ctypes.POINTER(ctypes.c_uint32)
+_u8p = ctypes.POINTER(ctypes.c_uint8)
+_f64p = ctypes.POINTER(ctypes.c_double)
+_i64p = ctypes.POINTER(ctypes.c_int64)
+_szp = ctypes.POINTER(ctypes.c_size_t)
+_i32p = ctypes.POINTER(ctypes.c_int32)
+
+MATH_FN = ctypes.CFUNCTYPE(ctypes.c_int, ctypes.c_void_p, _i32p, ctypes.c_size_t)
+
+_SIGNATURES = {
+    "qs_last_error": (ctypes.c_char_p, []),
+    "qs_abi_version": (ctypes.c_int, []),
+    "qs_create": (
+        ctypes.c_int,
+        [ctypes.c_uint32, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_uint32, ctypes.POINTER(ctypes.c_void_p)],
+    ),
+    "qs_destroy": (None, [ctypes.c_void_p]),
+    "qs_allocate_qubit": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32]),
+    "qs_deallocate_qubit": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32]),
+    "qs_get_classical_value": (
+        ctypes.c_int,
+        [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_double, ctypes.POINTER(ctypes.c_int)],
+    ),
+    "qs_is_classical": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_double, ctypes.POINTER(ctypes.c_int)]),
+    "qs_measure_qubits": (ctypes.c_int, [ctypes.c_void_p, _u32p, ctypes.c_size_t, _u8p]),
+    "qs_apply_controlled_gate": (
+        ctypes.c_int,
+        [ctypes.c_void_p, _f64p, _u32p, ctypes.c_size_t, _u32p, ctypes.c_size_t],
+    ),
+    "qs_emulate_math": (
+        ctypes.c_int,
+        [ctypes.c_void_p, MATH_FN, ctypes.c_void_p, _u32p, _u32p, ctypes.c_size_t, _u32p, ctypes.c_size_t],
+    ),
+    "qs_emulate_math_op": (
+        ctypes.c_int,
+        [ctypes.c_void_p, ctypes.c_int, _i64p, ctypes.c_size_t, _u32p, _u32p, ctypes.c_size_t, _u32p, ctypes.c_size_t],
+    ),
+    "qs_get_expectation_value": (
+        ctypes.c_int,
+        [ctypes.c_void_p, _u32p, _u8p, _szp, ctypes.c_size_t, _f64p, _u32p, ctypes.c_size_t, _f64p],
+    ),
+    "qs_apply_qubit_operator": (
+        ctypes.c_int,
+        [ctypes.c_void_p, _u32p, _u8p, _szp, ctypes.c_size_t, _f64p, _u32p, ctypes.c_size_t],
+    ),
+    "qs_emulate_time_evolution": (
+        ctypes.c_int,
+        [
+            ctypes.c_void_p,
+            _u32p,
+            _u8p,
+            _szp,
+            ctypes.c_size_t,
+            _f64p,
+            ctypes.c_double,
+            _u32p,
+            ctypes.c_size_t,
+            _u32p,
+            ctypes.c_size_t,
+        ],
+    ),
+    "qs_get_probability": (ctypes.c_int, [ctypes.c_void_p, _u8p, _u32p, ctypes.c_size_t, _f64p]),
+    "qs_get_amplitude": (ctypes.c_int, [ctypes.c_void_p, _u8p, _u32p, ctypes.c_size_t, _f64p]),
+    "qs_set_wavefunction": (ctypes.c_int, [ctypes.c_void_p, _f64p, ctypes.c_size_t, _u32p, ctypes.c_size_t]),
+    "qs_collapse_wavefunction": (ctypes.c_int, [ctypes.c_void_p, _u32p, ctypes.c_size_t, _u8p, ctypes.c_size_t]),
+    "qs_run": (ctypes.c_int, [ctypes.c_void_p]),
+    "qs_cheat_sizes": (ctypes.c_int, [ctypes.c_void_p, _szp, _szp]),
+    "qs_cheat": (ctypes.c_int, [ctypes.c_void_p, _u32p, _u32p, _f64p]),
+}
+
+#: every symbol the C ABI declares (include/projectq_amd.h)
+ABI_SYMBOLS = tuple(_SIGNATURES)
+
+
+def bind_library(lib):
+    """Set argtypes/restype for every C-ABI symbol of `lib` (a ctypes.CDLL); returns `lib`."""
+    for name, (res, args) in _SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    return lib
+
+
+def _ptr(arr, ctype):
+    return arr.ctypes.data_as(ctypes.POINTER(ctype))
+
+
+def _u32(seq):
+    arr = np.ascontiguousarray(np.asarray(seq, dtype=np.int64).reshape(-1))
+    if arr.size and (arr.min() < 0 or arr.max() > 0xFFFFFFFF):
+        raise TypeError("qubit ids must be unsigned 32-bit integers")
+    return arr.astype(np.uint32)
+
+
+def _bits(seq):
+    """Booleans as uint8; '0'/'1' characters count as their digit (_simulator.py:221, :247)."""
+    return np.ascontiguousarray(
+        np.array([1 if (bool(int(b)) if isinstance(b, str) else bool(b)) else 0 for b in seq], dtype=np.uint8)
+    )
+
+
+def _pauli_terms(terms, complex_coeffs):
+    """Flatten [(list[(idx, 'X'|'Y'|'Z')], coeff)] into the C-ABI arrays."""
+    idx, ops, offsets, coeffs = [], [], [0], []
+    for term, coeff in terms:
+        for local, op in term:
+            op = str(op)
+            if op not in ("X", "Y", "Z"):
+                raise ValueError(f"Pauli operator must be 'X', 'Y' or 'Z', got {op!r}")
+            idx.append(int(local))
+            ops.append(ord(op))
+        offsets.append(len(idx))
+        if complex_coeffs:
+            c = complex(coeff)
+            coeffs.extend([c.real, c.imag])
+        else:
+            if isinstance(coeff, complex):
+                raise TypeError("coefficients must be real for this method")
+            coeffs.append(float(coeff))
+    return (
+        _u32(idx),
+        np.asarray(ops, dtype=np.uint8),
+        np.asarray(offsets, dtype=np.uintp),
+        np.asarray(coeffs, dtype=np.float64),
+    )
+
+
+def _regs(quregs):
+    flat, sizes = [], []
+    for reg in quregs:
+        reg = list(reg)
+        flat.extend(reg)
+        sizes.append(len(reg))
+    return _u32(flat), np.asarray(sizes, dtype=np.uint32)
+
+
+class CAbiSimulator:
+    """The `_cppsim.Simulator` method surface over a C-ABI library."""
+
+    def __init__(self, lib, seed, precision="c128", fusion="reference", planner="greedy", capacity_qubits=0):
+        self._lib = lib
+        self._handle = None
+        self.precision = precision
+        self.fusion = fusion
+        self.planner = planner
+        handle = ctypes.c_void_p()
+        try:
+            prec = PRECISIONS[precision]
+            fmode = FUSION_MODES[fusion]
+            plan = PLANNERS[planner]
+        except KeyError as err:
+            raise ValueError(f"unknown option {err}") from None
+        seed = int(seed)
+        if seed < 0 or seed > 0xFFFFFFFF:
+            raise TypeError("seed must be an unsigned 32-bit integer")
+        self._check(lib.qs_create(seed, prec, fmode, plan, int(capacity_qubits), ctypes.byref(handle)))
+        self._handle = handle
+        self._cb_error = None
+
+    # ------------------------------------------------------------------ plumbing
+    def _check(self, status):
+        if status == QS_OK:
+            return
+        msg = (self._lib.qs_last_error() or b"").decode(errors="replace")
+        if status == QS_ERR_CALLBACK and getattr(self, "_cb_error", None) is not None:
+            err, self._cb_error = self._cb_error, None
+            raise err
+        if status == QS_ERR_VALUE:
+            raise ValueError(msg)
+        if status == QS_ERR_MEMORY:
+            raise MemoryError(msg)
+        raise RuntimeError(msg)
+
+    def __del__(self):
+        handle = getattr(self, "_handle", None)
+        if handle is not None and handle.value:
+            self._lib.qs_destroy(handle)
+            self._handle = None
+
+    # ------------------------------------------------------------------ _cppsim surface
+    def allocate_qubit(self, qubit_id):
+        self._check(self._lib.qs_allocate_qubit(self._handle, int(qubit_id)))
+
+    def deallocate_qubit(self, qubit_id):
+        self._check(self._lib.qs_deallocate_qubit(self._handle, int(qubit_id)))
+
+    def get_classical_value(self, qubit_id, tol):
+        out = ctypes.c_int()
+        self._check(self._lib.qs_get_classical_value(self._handle, int(qubit_id), float(tol), ctypes.byref(out)))
+        return bool(out.value)
+
+    def is_classical(self, qubit_id, tol):
+        out = ctypes.c_int()
+        self._check(self._lib.qs_is_classical(self._handle, int(qubit_id), float(tol), ctypes.byref(out)))
+        return bool(out.value)
+
+    def measure_qubits(self, ids):
+        ids = _u32(ids)
+        out = np.zeros(len(ids), dtype=np.uint8)
+        self._check(self._lib.qs_measure_qubits(self._handle, _ptr(ids, ctypes.c_uint32), len(ids), _ptr(out, ctypes.c_uint8)))
+        return [bool(v) for v in out]
+
+    def apply_controlled_gate(self, m, ids, ctrl):
+        ids = _u32(ids)
+        ctrl = _u32(ctrl)
+        mat = np.ascontiguousarray(np.asarray(m, dtype=np.complex128))
+        dim = 1 << len(ids)
+        if mat.shape != (dim, dim):
+            raise ValueError(f"apply_controlled_gate: a gate on {len(ids)} qubits needs a {dim}x{dim} matrix, got {mat.shape}")
+        self._check(
+            self._lib.qs_apply_controlled_gate(
+                self._handle,
+                _ptr(mat.view(np.float64), ctypes.c_double),
+                _ptr(ids, ctypes.c_uint32),
+                len(ids),
+                _ptr(ctrl, ctypes.c_uint32),
+                len(ctrl),
+            )
+        )
+
+    def _math_op(self, opcode, params, quregs, ctrl):
+        flat, sizes = _regs(quregs)
+        ctrl = _u32(ctrl)
+        prm = np.asarray(params, dtype=np.int64) if params else np.zeros(1, dtype=np.int64)
+        self._check(
+            self._lib.qs_emulate_math_op(
+                self._handle,
+                int(opcode),
+                _ptr(prm, ctypes.c_int64),
+                len(params),
+                _ptr(flat, ctypes.c_uint32),
+                _ptr(sizes, ctypes.c_uint32),
+                len(sizes),
+                _ptr(ctrl, ctypes.c_uint32),
+                len(ctrl),
+            )
+        )
+
+    def emulate_math(self, f, quregs, ctrl):
+        """simulator.hpp:224-269 through _cppsim.cpp:33-41.
+
+        Functions of projectq.libs.math (and of projectq_amd.gates) are recognised and run as
+        native integer permutations after a sampled cross-check against `f` itself; anything
+        else is called back once per distinct tuple of register values.
+        """
+        quregs = [list(r) for r in quregs]
+        recognised = _mathfuncs.recognise(f, quregs)
+        if recognised is not None:
+            opcode, params = recognised
+            self._math_op(opcode, params, quregs, ctrl)
+            return
+        flat, sizes = _regs(quregs)
+        ctrl = _u32(ctrl)
+        self._cb_error = None
+
+        def trampoline(_user, values, n_regs):
+            try:
+                res = f([int(values[i]) for i in range(n_regs)])
+                res = list(res)
+                if len(res) < n_regs:
+                    raise ValueError("emulate_math: function returned fewer values than registers")
+                for i in range(n_regs):
+                    v = int(res[i])
+                    if v < -(2**31) or v > 2**31 - 1:
+                        raise RuntimeError("Unable to cast Python instance to C++ type int")
+                    values[i] = v
+                return 0
+            except BaseException as err:  # noqa: BLE001 -- re-raised by _check
+                self._cb_error = err
+                return 1
+
+        cb = MATH_FN(trampoline)
+        self._check(
+            self._lib.qs_emulate_math(
+                self._handle,
+                cb,
+                None,
+                _ptr(flat, ctypes.c_uint32),
+                _ptr(sizes, ctypes.c_uint32),
+                len(sizes),
+                _ptr(ctrl, ctypes.c_uint32),
+                len(ctrl),
+            )
+        )
+
+    def emulate_math_addConstant(self, a, quregs, ctrl):  # noqa: N802 (reference name)
+        self._math_op(_mathfuncs.ADD_CONSTANT, [int(a)], quregs, ctrl)
+
+    def emulate_math_addConstantModN(self, a, N, quregs, ctrl):  # noqa: N802,N803
+        self._math_op(_mathfuncs.ADD_CONSTANT_MOD_N, [int(a), int(N)], quregs, ctrl)
+
+    def emulate_math_multiplyByConstantModN(self, a, N, quregs, ctrl):  # noqa: N802,N803
+        self._math_op(_mathfuncs.MULTIPLY_BY_CONSTANT_MOD_N, [int(a), int(N)], quregs, ctrl)
+
+    def get_expectation_value(self, terms, ids):
+        idx, ops, offs, coeffs = _pauli_terms(terms, complex_coeffs=False)
+        ids = _u32(ids)
+        out = ctypes.c_double()
+        self._check(
+            self._lib.qs_get_expectation_value(
+                self._handle,
+                _ptr(idx, ctypes.c_uint32),
+                _ptr(ops, ctypes.c_uint8),
+                _ptr(offs, ctypes.c_size_t),
+                len(offs) - 1,
+                _ptr(coeffs, ctypes.c_double),
+                _ptr(ids, ctypes.c_uint32),
+                len(ids),
+                ctypes.byref(out),
+            )
+        )
+        return out.value
+
+    def apply_qubit_operator(self, terms, ids):
+        idx, ops, offs, coeffs = _pauli_terms(terms, complex_coeffs=True)
+        ids = _u32(ids)
+        self._check(
+            self._lib.qs_apply_qubit_operator(
+                self._handle,
+                _ptr(idx, ctypes.c_uint32),
+                _ptr(ops, ctypes.c_uint8),
+                _ptr(offs, ctypes.c_size_t),
+                len(offs) - 1,
+                _ptr(coeffs, ctypes.c_double),
+                _ptr(ids, ctypes.c_uint32),
+                len(ids),
+            )
+        )
+
+    def emulate_time_evolution(self, terms, time, ids, ctrl):
+        idx, ops, offs, coeffs = _pauli_terms(terms, complex_coeffs=False)
+        ids = _u32(ids)
+        ctrl = _u32(ctrl)
+        self._check(
+            self._lib.qs_emulate_time_evolution(
+                self._handle,
+                _ptr(idx, ctypes.c_uint32),
+                _ptr(ops, ctypes.c_uint8),
+                _ptr(offs, ctypes.c_size_t),
+                len(offs) - 1,
+                _ptr(coeffs, ctypes.c_double),
+                float(time),
+                _ptr(ids, ctypes.c_uint32),
+                len(ids),
+                _ptr(ctrl, ctypes.c_uint32),
+                len(ctrl),
+            )
+        )
+
+    def get_probability(self, bit_string, ids):
+        bits = _bits(bit_string)
+        ids = _u32(ids)
+        if len(bits) < len(ids):
+            raise ValueError("get_probability: fewer bits than qubit ids")
+        out = ctypes.c_double()
+        self._check(
+            self._lib.qs_get_probability(
+                self._handle, _ptr(bits, ctypes.c_uint8), _ptr(ids, ctypes.c_uint32), len(ids), ctypes.byref(out)
+            )
+        )
+        return out.value
+
+    def get_amplitude(self, bit_string, ids):
+        bits = _bits(bit_string)
+        ids = _u32(ids)
+        if len(bits) < len(ids):
+            raise ValueError("get_amplitude: fewer bits than qubit ids")
+        out = np.zeros(2, dtype=np.float64)
+        self._check(
+            self._lib.qs_get_amplitude(
+                self._handle, _ptr(bits, ctypes.c_uint8), _ptr(ids, ctypes.c_uint32), len(ids), _ptr(out, ctypes.c_double)
+            )
+        )
+        return complex(out[0], out[1])
+
+    def set_wavefunction(self, wavefunction, ordering):
+        wf = np.ascontiguousarray(np.asarray(wavefunction, dtype=np.complex128).reshape(-1))
+        ordering = _u32(ordering)
+        self._check(
+            self._lib.qs_set_wavefunction(
+                self._handle, _ptr(wf.view(np.float64), ctypes.c_double), len(wf), _ptr(ordering, ctypes.c_uint32), len(ordering)
+            )
+        )
+
+    def collapse_wavefunction(self, ids, values):
+        ids = _u32(ids)
+        vals = _bits(values)
+        self._check(
+            self._lib.qs_collapse_wavefunction(
+                self._handle, _ptr(ids, ctypes.c_uint32), len(ids), _ptr(vals, ctypes.c_uint8), len(vals)
+            )
+        )
+
+    def run(self):
+        self._check(self._lib.qs_run(self._handle))
+
+    def cheat_numpy(self):
+        """cheat() without the Python-list conversion: (dict id->position, complex128 ndarray)."""
+        nq = ctypes.c_size_t()
+        na = ctypes.c_size_t()
+        self._check(self._lib.qs_cheat_sizes(self._handle, ctypes.byref(nq), ctypes.byref(na)))
+        ids = np.zeros(max(nq.value, 1), dtype=np.uint32)
+        pos = np.zeros(max(nq.value, 1), dtype=np.uint32)
+        state = np.zeros(na.value, dtype=np.complex128)
+        self._check(
+            self._lib.qs_cheat(
+                self._handle, _ptr(ids, ctypes.c_uint32), _ptr(pos, ctypes.c_uint32), _ptr(state.view(np.float64), ctypes.c_double)
+            )
+        )
+        mapping = {int(ids[i]): int(pos[i]) for i in range(nq.value)}
+        return mapping, state
+
+    def cheat(self):
+        """simulator.hpp:529-532: (dict id -> bit position, list of complex amplitudes)."""
+        mapping, state = self.cheat_numpy()
+        return mapping, state.tolist()
+
+
+def qubit_count_for(n_amps):
+    """log2 of a power-of-two amplitude count."""
+    n = int(math.log2(n_amps))
+    if 1 << n != n_amps:
+        raise ValueError("amplitude count must be a power of two")
+    return n

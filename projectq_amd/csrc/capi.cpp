@@ -1,0 +1,206 @@
+// extern "C" boundary of the simulator library: one entry point per method of the reference's
+// pybind11 class (projectq/backends/_sim/_cppsim.cpp:43-67); declarations and conventions in
+// include/projectq_amd.h.  Exceptions become status codes (the reference's pybind11 exception
+// translation: runtime_error -> RuntimeError, length_error/invalid_argument -> ValueError,
+// bad_alloc -> MemoryError).
+#include "projectq_amd.h"
+
+#include <cstring>
+#include <string>
+
+#include "engine.hpp"
+
+struct qs_simulator {
+  qs::Engine engine;
+  int device;
+  qs_simulator(uint32_t seed, int prec, int fusion, int planner, unsigned hint, int dev)
+      : engine(seed, prec, fusion, planner, hint), device(dev) {}
+};
+
+namespace {
+
+thread_local std::string g_last_error;
+
+template <class F>
+int guarded(qs_simulator *s, F &&f) {
+  try {
+    if (s) QS_HIP(hipSetDevice(s->device));
+    f();
+    return QS_OK;
+  } catch (const qs::CallbackError &e) {
+    g_last_error = e.what();
+    return QS_ERR_CALLBACK;
+  } catch (const qs::DeviceError &e) {
+    g_last_error = e.what();
+    return QS_ERR_DEVICE;
+  } catch (const std::length_error &e) {
+    g_last_error = e.what();
+    return QS_ERR_VALUE;
+  } catch (const std::invalid_argument &e) {
+    g_last_error = e.what();
+    return QS_ERR_VALUE;
+  } catch (const std::bad_alloc &) {
+    g_last_error = "out of device or host memory";
+    return QS_ERR_MEMORY;
+  } catch (const std::exception &e) {
+    g_last_error = e.what();
+    return QS_ERR_RUNTIME;
+  } catch (...) {
+    g_last_error = "unknown error";
+    return QS_ERR_RUNTIME;
+  }
+}
+
+std::vector<unsigned> ids_of(const uint32_t *p, size_t n) { return std::vector<unsigned>(p, p + n); }
+
+std::vector<qs::PauliString> terms_of(const uint32_t *idx, const uint8_t *ops, const size_t *off, size_t nt) {
+  std::vector<qs::PauliString> t(nt);
+  for (size_t i = 0; i < nt; ++i)
+    for (size_t j = off[i]; j < off[i + 1]; ++j) t[i].ops.push_back({idx[j], (char)ops[j]});
+  return t;
+}
+
+std::vector<std::vector<unsigned>> regs_of(const uint32_t *q, const uint32_t *sz, size_t nr) {
+  std::vector<std::vector<unsigned>> r(nr);
+  size_t k = 0;
+  for (size_t i = 0; i < nr; ++i)
+    for (uint32_t j = 0; j < sz[i]; ++j) r[i].push_back(q[k++]);
+  return r;
+}
+
+}  // namespace
+
+#define API extern "C" __attribute__((visibility("default")))
+
+API const char *qs_last_error(void) { return g_last_error.c_str(); }
+API int qs_abi_version(void) { return 1; }
+
+API int qs_create(uint32_t seed, int precision, int fusion_mode, int planner, uint32_t capacity_hint_qubits,
+                  qs_simulator **out) {
+  return guarded(nullptr, [&] {
+    int dev = 0;
+    QS_HIP(hipGetDevice(&dev));
+    *out = new qs_simulator(seed, precision, fusion_mode, planner, capacity_hint_qubits, dev);
+  });
+}
+
+API void qs_destroy(qs_simulator *s) {
+  if (!s) return;
+  (void)hipSetDevice(s->device);
+  delete s;
+}
+
+API int qs_allocate_qubit(qs_simulator *s, uint32_t id) {
+  return guarded(s, [&] { s->engine.allocate_qubit(id); });
+}
+API int qs_deallocate_qubit(qs_simulator *s, uint32_t id) {
+  return guarded(s, [&] { s->engine.deallocate_qubit(id); });
+}
+API int qs_get_classical_value(qs_simulator *s, uint32_t id, double tol, int *value) {
+  return guarded(s, [&] { *value = s->engine.get_classical_value(id, tol) ? 1 : 0; });
+}
+API int qs_is_classical(qs_simulator *s, uint32_t id, double tol, int *value) {
+  return guarded(s, [&] { *value = s->engine.is_classical(id, tol) ? 1 : 0; });
+}
+API int qs_measure_qubits(qs_simulator *s, const uint32_t *ids, size_t n, uint8_t *results) {
+  return guarded(s, [&] {
+    auto r = s->engine.measure_qubits(ids_of(ids, n));
+    std::memcpy(results, r.data(), r.size());
+  });
+}
+API int qs_apply_controlled_gate(qs_simulator *s, const double *matrix, const uint32_t *ids, size_t n_ids,
+                                 const uint32_t *ctrl, size_t n_ctrl) {
+  return guarded(s, [&] {
+    if (n_ids > 12) throw qs::ValueError("apply_controlled_gate: gate too large");
+    qs::Gate g;
+    const size_t d = size_t(1) << n_ids;
+    g.m.resize(d * d);
+    for (size_t i = 0; i < d * d; ++i) g.m[i] = qs::cplx(matrix[2 * i], matrix[2 * i + 1]);
+    g.ids = ids_of(ids, n_ids);
+    g.ctrl = ids_of(ctrl, n_ctrl);
+    s->engine.apply_controlled_gate(std::move(g));
+  });
+}
+API int qs_emulate_math(qs_simulator *s, qs_math_fn fn, void *user, const uint32_t *reg_qubits,
+                        const uint32_t *reg_sizes, size_t n_regs, const uint32_t *ctrl, size_t n_ctrl) {
+  return guarded(s, [&] {
+    s->engine.emulate_math_callback([&](int32_t *v, size_t nr) { return fn(user, v, nr) == 0; },
+                                    regs_of(reg_qubits, reg_sizes, n_regs), ids_of(ctrl, n_ctrl));
+  });
+}
+API int qs_emulate_math_op(qs_simulator *s, int opcode, const int64_t *params, size_t n_params,
+                           const uint32_t *reg_qubits, const uint32_t *reg_sizes, size_t n_regs,
+                           const uint32_t *ctrl, size_t n_ctrl) {
+  return guarded(s, [&] {
+    if (opcode < QS_MATH_ADD_CONSTANT || opcode > QS_MATH_INVERSE_DIVIDE_QUANTUM)
+      throw qs::ValueError("emulate_math: unknown opcode");
+    std::vector<int64_t> prm(params, params + n_params);
+    if (opcode <= QS_MATH_MULTIPLY_BY_CONSTANT_MOD_N) {
+      size_t need = opcode == QS_MATH_ADD_CONSTANT ? 1 : 2;
+      if (n_params < need) throw qs::ValueError("emulate_math: missing constant parameters");
+      if (opcode != QS_MATH_ADD_CONSTANT && (int32_t)prm[1] == 0) throw qs::ValueError("emulate_math: N must be non-zero");
+    } else {
+      size_t need = (opcode == QS_MATH_ADD_QUANTUM || opcode == QS_MATH_INVERSE_ADD_QUANTUM ||
+                     opcode == QS_MATH_SUBTRACT_QUANTUM) ? 2 : 3;
+      if (n_regs < need || n_regs > 3) throw qs::ValueError("emulate_math: wrong number of registers for this function");
+    }
+    s->engine.emulate_math_op(opcode, prm, regs_of(reg_qubits, reg_sizes, n_regs), ids_of(ctrl, n_ctrl));
+  });
+}
+API int qs_get_expectation_value(qs_simulator *s, const uint32_t *term_idx, const uint8_t *term_ops,
+                                 const size_t *term_offsets, size_t n_terms, const double *coeffs,
+                                 const uint32_t *ids, size_t n_ids, double *result) {
+  return guarded(s, [&] {
+    *result = s->engine.get_expectation_value(terms_of(term_idx, term_ops, term_offsets, n_terms),
+                                              std::vector<double>(coeffs, coeffs + n_terms), ids_of(ids, n_ids));
+  });
+}
+API int qs_apply_qubit_operator(qs_simulator *s, const uint32_t *term_idx, const uint8_t *term_ops,
+                                const size_t *term_offsets, size_t n_terms, const double *coeffs,
+                                const uint32_t *ids, size_t n_ids) {
+  return guarded(s, [&] {
+    std::vector<qs::cplx> c(n_terms);
+    for (size_t i = 0; i < n_terms; ++i) c[i] = qs::cplx(coeffs[2 * i], coeffs[2 * i + 1]);
+    s->engine.apply_qubit_operator(terms_of(term_idx, term_ops, term_offsets, n_terms), c, ids_of(ids, n_ids));
+  });
+}
+API int qs_emulate_time_evolution(qs_simulator *s, const uint32_t *term_idx, const uint8_t *term_ops,
+                                  const size_t *term_offsets, size_t n_terms, const double *coeffs, double time,
+                                  const uint32_t *ids, size_t n_ids, const uint32_t *ctrl, size_t n_ctrl) {
+  return guarded(s, [&] {
+    s->engine.emulate_time_evolution(terms_of(term_idx, term_ops, term_offsets, n_terms),
+                                     std::vector<double>(coeffs, coeffs + n_terms), time, ids_of(ids, n_ids),
+                                     ids_of(ctrl, n_ctrl));
+  });
+}
+API int qs_get_probability(qs_simulator *s, const uint8_t *bits, const uint32_t *ids, size_t n, double *result) {
+  return guarded(s, [&] {
+    *result = s->engine.get_probability(std::vector<uint8_t>(bits, bits + n), ids_of(ids, n));
+  });
+}
+API int qs_get_amplitude(qs_simulator *s, const uint8_t *bits, const uint32_t *ids, size_t n, double *result) {
+  return guarded(s, [&] {
+    qs::cplx a = s->engine.get_amplitude(std::vector<uint8_t>(bits, bits + n), ids_of(ids, n));
+    result[0] = a.real();
+    result[1] = a.imag();
+  });
+}
+API int qs_set_wavefunction(qs_simulator *s, const double *wf, size_t n_amps, const uint32_t *ordering,
+                            size_t n_ids) {
+  return guarded(s, [&] { s->engine.set_wavefunction(wf, n_amps, ids_of(ordering, n_ids)); });
+}
+API int qs_collapse_wavefunction(qs_simulator *s, const uint32_t *ids, size_t n_ids, const uint8_t *values,
+                                 size_t n_values) {
+  return guarded(s, [&] {
+    s->engine.collapse_wavefunction(ids_of(ids, n_ids), std::vector<uint8_t>(values, values + n_values));
+  });
+}
+API int qs_run(qs_simulator *s) {
+  return guarded(s, [&] { s->engine.run(); });
+}
+API int qs_cheat_sizes(qs_simulator *s, size_t *n_qubits, size_t *n_amps) {
+  return guarded(s, [&] { s->engine.cheat_sizes(n_qubits, n_amps); });
+}
+API int qs_cheat(qs_simulator *s, uint32_t *ids, uint32_t *positions, double *state) {
+  return guarded(s, [&] { s->engine.cheat(ids, positions, state); });
+}

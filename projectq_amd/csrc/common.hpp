@@ -1,0 +1,61 @@
+// Shared types and error plumbing of the MI355X simulator library.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <new>
+#include <stdexcept>
+#include <string>
+
+namespace qs {
+
+// Error classes surfaced through the C ABI (QS_ERR_* in include/projectq_amd.h).
+struct ValueError : std::invalid_argument {
+  using std::invalid_argument::invalid_argument;
+};
+struct RuntimeError : std::runtime_error {
+  using std::runtime_error::runtime_error;
+};
+struct DeviceError : std::runtime_error {
+  using std::runtime_error::runtime_error;
+};
+struct CallbackError : std::runtime_error {
+  using std::runtime_error::runtime_error;
+};
+
+inline void hip_check(hipError_t e, const char *what) {
+  if (e == hipSuccess) return;
+  if (e == hipErrorOutOfMemory || e == hipErrorMemoryAllocation) {
+    (void)hipGetLastError();
+    throw std::bad_alloc();
+  }
+  throw DeviceError(std::string(what) + ": " + hipGetErrorString(e));
+}
+#define QS_HIP(expr) ::qs::hip_check((expr), #expr)
+
+// Amplitude types: interleaved (re, im).
+template <typename R>
+struct Amp;
+template <>
+struct Amp<double> {
+  using type = double2;
+};
+template <>
+struct Amp<float> {
+  using type = float2;
+};
+template <typename R>
+using amp_t = typename Amp<R>::type;
+
+// Largest number of bit positions a kernel argument can name (targets + controls + ...).
+constexpr int kMaxBits = 48;
+
+// Sorted list of bit positions at which a zero is inserted into a compact counter; used to
+// enumerate every index whose chosen bits are fixed (targets 0, controls 1, ...).
+struct BitInsert {
+  uint32_t count;
+  uint8_t pos[kMaxBits];
+};
+
+}  // namespace qs

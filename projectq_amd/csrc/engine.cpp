@@ -1,0 +1,731 @@
+// Engine implementation: host logic of every simulator method, restating
+// projectq/backends/_sim/_cppkernels/simulator.hpp method by method, with the data-parallel
+// loops replaced by the kernels of kernels.hip.  The state never leaves HBM except through
+// cheat() and get_amplitude().
+#include "engine.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <string>
+
+namespace qs {
+
+namespace {
+
+BitInsert make_insert(std::vector<uint32_t> positions) {
+  std::sort(positions.begin(), positions.end());
+  if (positions.size() > (size_t)kMaxBits) throw ValueError("too many fixed qubits in one call");
+  BitInsert ins{};
+  ins.count = (uint32_t)positions.size();
+  for (size_t i = 0; i < positions.size(); ++i) ins.pos[i] = (uint8_t)positions[i];
+  return ins;
+}
+
+std::vector<uint32_t> bits_of(uint64_t mask) {
+  std::vector<uint32_t> out;
+  for (uint32_t p = 0; p < 64; ++p)
+    if ((mask >> p) & 1) out.push_back(p);
+  return out;
+}
+
+}  // namespace
+
+Engine::Engine(uint32_t seed, int precision, int fusion_mode, int planner, unsigned capacity_hint)
+    : c64_(precision == 1), fusion_mode_(fusion_mode), planner_kind_(planner),
+      cap_hint_(capacity_hint), rng_(seed) {
+  if (precision != 0 && precision != 1) throw ValueError("precision must be QS_C128 or QS_C64");
+  if (fusion_mode != 0 && fusion_mode != 1) throw ValueError("unknown fusion mode");
+  if (planner != 0 && planner != 1) throw ValueError("unknown planner");
+  if (capacity_hint > 36) throw ValueError("capacity hint above 36 qubits");
+  QS_HIP(hipGetDevice(&device_));
+  QS_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+  QS_HIP(hipMalloc(&d_partials_, sizeof(double) * 2 * dev::kReduceGrid));
+  QS_HIP(hipMalloc(&d_result_, sizeof(double) * 4));
+  QS_HIP(hipHostMalloc((void **)&h_result_, sizeof(double) * 4, hipHostMallocDefault));
+  QS_HIP(hipMalloc(&d_flags_, sizeof(int) * 4));
+  QS_HIP(hipHostMalloc((void **)&h_flags_, sizeof(int) * 4, hipHostMallocDefault));
+  QS_HIP(hipMalloc(&d_key_, sizeof(unsigned long long)));
+  QS_HIP(hipHostMalloc((void **)&h_key_, sizeof(unsigned long long), hipHostMallocDefault));
+  QS_HIP(hipMalloc(&d_arena_, kArenaBytes));
+  QS_HIP(hipHostMalloc((void **)&h_arena_, kArenaBytes, hipHostMallocDefault));
+  // initial state |> = [1]  (simulator.hpp:48-51)
+  ensure_capacity(1);
+  if (c64_) {
+    float one[2] = {1.0f, 0.0f};
+    QS_HIP(hipMemcpy(psi_, one, sizeof(one), hipMemcpyHostToDevice));
+  } else {
+    double one[2] = {1.0, 0.0};
+    QS_HIP(hipMemcpy(psi_, one, sizeof(one), hipMemcpyHostToDevice));
+  }
+}
+
+Engine::~Engine() {
+  if (stream_) (void)hipStreamSynchronize(stream_);
+  (void)hipFree(psi_);
+  release_scratch();
+  (void)hipFree(d_partials_);
+  (void)hipFree(d_result_);
+  (void)hipHostFree(h_result_);
+  (void)hipFree(d_flags_);
+  (void)hipHostFree(h_flags_);
+  (void)hipFree(d_key_);
+  (void)hipHostFree(h_key_);
+  (void)hipFree(d_chunks_);
+  (void)hipHostFree(h_chunks_);
+  (void)hipFree(d_arena_);
+  (void)hipHostFree(h_arena_);
+  if (stream_) (void)hipStreamDestroy(stream_);
+}
+
+// ------------------------------------------------------------------------------ memory
+void Engine::release_scratch() {
+  for (auto &b : buf_) {
+    (void)hipFree(b);
+    b = nullptr;
+  }
+}
+
+void Engine::ensure_capacity(uint64_t need) {
+  if (need <= cap_) return;
+  uint64_t cap = std::max<uint64_t>(need, cap_hint_ ? (uint64_t(1) << cap_hint_) : 1);
+  void *fresh = nullptr;
+  QS_HIP(hipMalloc(&fresh, cap * amp_bytes()));
+  if (psi_) {
+    QS_HIP(hipMemcpyAsync(fresh, psi_, amps() * amp_bytes(), hipMemcpyDeviceToDevice, stream_));
+    sync();
+    QS_HIP(hipFree(psi_));
+  }
+  release_scratch();  // scratch follows the new capacity lazily
+  psi_ = fresh;
+  cap_ = cap;
+}
+
+void *Engine::scratch(int which) {
+  if (!buf_[which]) QS_HIP(hipMalloc(&buf_[which], cap_ * amp_bytes()));
+  return buf_[which];
+}
+
+const void *Engine::upload(const void *host, size_t bytes) {
+  size_t need = (bytes + 255) & ~size_t(255);
+  if (need > kArenaBytes) throw ValueError("argument too large for the upload arena");
+  if (arena_off_ + need > kArenaBytes) {
+    sync();  // every earlier copy out of the pinned arena has completed
+    arena_off_ = 0;
+  }
+  std::memcpy(h_arena_ + arena_off_, host, bytes);
+  QS_HIP(hipMemcpyAsync(d_arena_ + arena_off_, h_arena_ + arena_off_, bytes, hipMemcpyHostToDevice, stream_));
+  const void *d = d_arena_ + arena_off_;
+  arena_off_ += need;
+  return d;
+}
+
+void Engine::sync() { QS_HIP(hipStreamSynchronize(stream_)); }
+
+double Engine::finish(int n_partials, int width, double *second) {
+  dev::finish_sum(d_partials_, n_partials, width, d_result_, stream_);
+  QS_HIP(hipMemcpyAsync(h_result_, d_result_, sizeof(double) * width, hipMemcpyDeviceToHost, stream_));
+  sync();
+  if (second) *second = h_result_[1];
+  return h_result_[0];
+}
+
+void Engine::check_known(const std::vector<unsigned> &ids, const char *what) const {
+  for (unsigned q : ids)
+    if (!pos_.count(q)) throw RuntimeError(std::string(what) + ": Unknown qubit id.");
+}
+
+uint64_t Engine::mask_of(const std::vector<unsigned> &ids) const {
+  uint64_t m = 0;
+  for (unsigned q : ids) m |= uint64_t(1) << pos_.at(q);
+  return m;
+}
+
+double Engine::masked_norm(uint64_t mask, uint64_t val) {
+  BitInsert ins = make_insert(bits_of(mask));
+  uint64_t n_sub = amps() >> ins.count;
+  dispatch([&](auto tag) {
+    using R = decltype(tag);
+    dev::norm_masked<R>((const amp_t<R> *)psi_, n_sub, ins, val, d_partials_, stream_);
+  });
+  return finish(dev::kReduceGrid, 1);
+}
+
+// ------------------------------------------------------------------------------ allocation
+// simulator.hpp:55-74: the new qubit takes the top bit; the old state is the lower half and
+// the upper half starts at zero.  Capacity is kept, so allocation is one memset.
+void Engine::allocate_qubit(unsigned id) {
+  if (pos_.count(id)) throw RuntimeError("AllocateQubit: ID already exists. Qubit IDs should be unique.");
+  if (n_ >= 40) throw ValueError("too many qubits");
+  ensure_capacity(amps() * 2);
+  QS_HIP(hipMemsetAsync((char *)psi_ + amps() * amp_bytes(), 0, amps() * amp_bytes(), stream_));
+  pos_[id] = n_++;
+}
+
+// simulator.hpp:93-108
+bool Engine::is_classical(unsigned id, double tol) {
+  run();
+  check_known({id}, "is_classical");
+  QS_HIP(hipMemsetAsync(d_flags_, 0, sizeof(int) * 2, stream_));
+  int p = (int)pos_[id];
+  dispatch([&](auto tag) {
+    using R = decltype(tag);
+    dev::classical_flags<R>((const amp_t<R> *)psi_, amps(), p, tol, d_flags_, stream_);
+  });
+  QS_HIP(hipMemcpyAsync(h_flags_, d_flags_, sizeof(int) * 2, hipMemcpyDeviceToHost, stream_));
+  sync();
+  return (h_flags_[0] ^ h_flags_[1]) == 1;
+}
+
+// simulator.hpp:76-91: the first pair (bit clear, bit set) in index order holding an entry
+// above tol decides; the bit-clear entry is looked at first.
+bool Engine::get_classical_value(unsigned id, double tol) {
+  run();
+  check_known({id}, "get_classical_value");
+  *h_key_ = ~0ull;
+  QS_HIP(hipMemcpyAsync(d_key_, h_key_, sizeof(unsigned long long), hipMemcpyHostToDevice, stream_));
+  int p = (int)pos_[id];
+  dispatch([&](auto tag) {
+    using R = decltype(tag);
+    dev::first_pair<R>((const amp_t<R> *)psi_, amps() / 2, p, tol, d_key_, stream_);
+  });
+  QS_HIP(hipMemcpyAsync(h_key_, d_key_, sizeof(unsigned long long), hipMemcpyDeviceToHost, stream_));
+  sync();
+  if (*h_key_ == ~0ull) return false;
+  return (*h_key_ & 1) == 1;
+}
+
+// simulator.hpp:194-202 with collapse_vector(shrink) :110-143
+void Engine::deallocate_qubit(unsigned id) {
+  run();
+  check_known({id}, "deallocate_qubit");
+  if (!is_classical(id, 1e-12))
+    throw RuntimeError(
+        "Error: Qubit has not been measured / uncomputed! There is most likely a bug in your code.");
+  const bool value = get_classical_value(id, 1e-12);
+  const unsigned p = pos_[id];
+  const uint64_t half = amps() / 2;
+  if (p == n_ - 1) {  // top bit: the kept half is already contiguous
+    if (value)
+      QS_HIP(hipMemcpyAsync(psi_, (char *)psi_ + half * amp_bytes(), half * amp_bytes(),
+                            hipMemcpyDeviceToDevice, stream_));
+  } else {
+    void *dst = scratch(0);
+    dispatch([&](auto tag) {
+      using R = decltype(tag);
+      dev::drop_bit<R>((const amp_t<R> *)psi_, (amp_t<R> *)dst, half, (int)p, value ? 1 : 0, stream_);
+    });
+    std::swap(psi_, buf_[0]);
+  }
+  sync();
+  for (auto &kv : pos_)
+    if (kv.second > p) kv.second--;
+  pos_.erase(id);
+  n_--;
+}
+
+// ------------------------------------------------------------------------------ measurement
+// simulator.hpp:145-186.  One uniform draw; the serial cumulative scan of the reference is
+// replaced by chunk sums (one read pass), a host prefix over the chunks and an in-chunk scan,
+// which lands on the same index unless the draw is within rounding of a boundary.
+std::vector<uint8_t> Engine::measure_qubits(const std::vector<unsigned> &ids) {
+  run();
+  check_known(ids, "measure_qubits");
+  const double rnd = uni_(rng_);
+  const uint64_t N = amps();
+  uint64_t pick;
+  if (!(rnd > 0.0)) {
+    pick = ~uint64_t(0);  // the reference's loop never runs and `pick--` wraps
+  } else {
+    const int lg = (int)std::min<unsigned>(n_, 12);
+    const uint64_t len = uint64_t(1) << lg, chunks = N >> lg;
+    if (chunks > chunks_cap_) {
+      (void)hipFree(d_chunks_);
+      (void)hipHostFree(h_chunks_);
+      d_chunks_ = h_chunks_ = nullptr;
+      chunks_cap_ = 0;
+      QS_HIP(hipMalloc(&d_chunks_, chunks * sizeof(double)));
+      QS_HIP(hipHostMalloc((void **)&h_chunks_, chunks * sizeof(double), hipHostMallocDefault));
+      chunks_cap_ = chunks;
+    }
+    dispatch([&](auto tag) {
+      using R = decltype(tag);
+      dev::chunk_norms<R>((const amp_t<R> *)psi_, N, lg, d_chunks_, stream_);
+    });
+    QS_HIP(hipMemcpyAsync(h_chunks_, d_chunks_, chunks * sizeof(double), hipMemcpyDeviceToHost, stream_));
+    sync();
+    double run_sum = 0.0;
+    uint64_t c = 0;
+    for (; c < chunks; ++c) {
+      if (run_sum + h_chunks_[c] >= rnd) break;
+      run_sum += h_chunks_[c];
+    }
+    if (c == chunks) {
+      pick = N - 1;  // the draw exceeds the total norm: the scan runs off the end
+    } else {
+      *h_key_ = ~0ull;
+      QS_HIP(hipMemcpyAsync(d_key_, h_key_, sizeof(unsigned long long), hipMemcpyHostToDevice, stream_));
+      dispatch([&](auto tag) {
+        using R = decltype(tag);
+        dev::chunk_pick<R>((const amp_t<R> *)psi_, c * len, len, run_sum, rnd, d_key_, stream_);
+      });
+      QS_HIP(hipMemcpyAsync(h_key_, d_key_, sizeof(unsigned long long), hipMemcpyDeviceToHost, stream_));
+      sync();
+      pick = (*h_key_ == ~0ull) ? (c + 1) * len - 1 : (uint64_t)*h_key_;
+    }
+  }
+  std::vector<uint8_t> res(ids.size());
+  uint64_t mask = 0, val = 0;
+  for (size_t i = 0; i < ids.size(); ++i) {
+    unsigned p = pos_[ids[i]];
+    res[i] = (uint8_t)((pick >> p) & 1);
+    mask |= uint64_t(1) << p;
+    val |= uint64_t(res[i]) << p;
+  }
+  double kept = masked_norm(mask, val);
+  double scale = 1.0 / std::sqrt(kept);
+  dispatch([&](auto tag) {
+    using R = decltype(tag);
+    dev::collapse<R>((amp_t<R> *)psi_, N, mask, val, scale, stream_);
+  });
+  return res;
+}
+
+// ------------------------------------------------------------------------------ gates
+// simulator.hpp:204-222 (reference fusion rule) or the window queue of the planners.
+void Engine::apply_controlled_gate(Gate g) {
+  check_known(g.ids, "apply_controlled_gate");
+  check_known(g.ctrl, "apply_controlled_gate");
+  if (g.m.size() != (size_t(1) << (2 * g.ids.size())))
+    throw ValueError("apply_controlled_gate: matrix size does not match the number of qubits");
+  if (fusion_mode_ == 1) {
+    window_.push_back(std::move(g));
+    return;
+  }
+  GateQueue trial = queue_;
+  trial.insert(g);
+  const unsigned now = trial.num_qubits(), before = queue_.num_qubits();
+  if (now >= 4 && now <= 5) {
+    queue_ = std::move(trial);
+    run();
+  } else if (now > 5 || (size_t)(now - g.ids.size()) > (size_t)before) {
+    run();
+    queue_.insert(g);
+  } else {
+    queue_ = std::move(trial);
+  }
+}
+
+// simulator.hpp:487-527: fuse the queue and stream the state once.
+void Engine::run() {
+  if (fusion_mode_ == 1) {
+    flush_window();
+    return;
+  }
+  if (queue_.empty()) return;
+  Block b = queue_.fuse();
+  exec_block(b);  // throws for > 5 qubits before the queue is cleared, like the reference
+  queue_.clear();
+}
+
+// Greedy planner over a window of queued gates: the reference's block rule (min 4 / max 5,
+// simulator.hpp:204-222) applied gate by gate; each block is one streaming pass.
+void Engine::flush_window() {
+  if (window_.empty()) return;
+  std::vector<Gate> gates;
+  gates.swap(window_);
+  GateQueue q;
+  for (const Gate &g : gates) {
+    GateQueue trial = q;
+    trial.insert(g);
+    const unsigned now = trial.num_qubits(), before = q.num_qubits();
+    if (now >= 4 && now <= 5) {
+      exec_block(trial.fuse());
+      q.clear();
+    } else if (now > 5 || (size_t)(now - g.ids.size()) > (size_t)before) {
+      if (!q.empty()) exec_block(q.fuse());
+      q.clear();
+      q.insert(g);
+    } else {
+      q = std::move(trial);
+    }
+  }
+  if (!q.empty()) exec_block(q.fuse());
+}
+
+void Engine::exec_block(const Block &b) {
+  const int k = (int)b.qubits.size();
+  if (k < 1 || k > 5) throw ValueError("Gates with more than 5 qubits are not supported!");
+  std::vector<uint32_t> tpos(k), fixed;
+  for (int l = 0; l < k; ++l) tpos[l] = pos_.at(b.qubits[l]);
+  uint64_t cmask = 0;
+  for (unsigned c : b.ctrl) cmask |= uint64_t(1) << pos_.at(c);
+  uint64_t tmask = 0;
+  for (uint32_t p : tpos) tmask |= uint64_t(1) << p;
+  if (tmask & cmask) return;  // a control that is also a target is never 1 on a base index
+  fixed = tpos;
+  for (uint32_t p : bits_of(cmask)) fixed.push_back(p);
+  BitInsert ins = make_insert(fixed);
+  if (ins.count > n_) return;
+  const uint64_t n_groups = amps() >> ins.count;
+  const size_t D = size_t(1) << k;
+  dispatch([&](auto tag) {
+    using R = decltype(tag);
+    std::vector<amp_t<R>> m(D * D);
+    for (size_t i = 0; i < D * D; ++i) {
+      m[i].x = (R)b.m[i].real();
+      m[i].y = (R)b.m[i].imag();
+    }
+    const amp_t<R> *dm = (const amp_t<R> *)upload(m.data(), m.size() * sizeof(amp_t<R>));
+    dev::apply_block<R>((amp_t<R> *)psi_, n_groups, ins, cmask, tpos.data(), k, dm, stream_);
+  });
+}
+
+// ------------------------------------------------------------------------------ emulate_math
+dev::MathArgs Engine::math_args(const std::vector<std::vector<unsigned>> &regs,
+                                const std::vector<unsigned> &ctrl) const {
+  if (regs.size() > (size_t)dev::kMaxRegs) throw ValueError("emulate_math: at most 8 registers");
+  dev::MathArgs a{};
+  a.n_regs = (int32_t)regs.size();
+  for (size_t r = 0; r < regs.size(); ++r) {
+    if (regs[r].size() > 32) throw ValueError("emulate_math: registers hold at most 32 qubits");
+    a.width[r] = (uint8_t)regs[r].size();
+    for (size_t b = 0; b < regs[r].size(); ++b) a.pos[r][b] = (uint8_t)pos_.at(regs[r][b]);
+  }
+  a.cmask = mask_of(ctrl);
+  return a;
+}
+
+void Engine::math_apply(dev::MathArgs args, const int32_t *d_table, bool known_bijective) {
+  const uint64_t N = amps();
+  QS_HIP(hipMemsetAsync(d_flags_, 0, sizeof(int) * 2, stream_));
+  bool injective = known_bijective;
+  if (!injective) {
+    const size_t words = (N + 31) / 32;
+    uint32_t *bitmap = nullptr;
+    QS_HIP(hipMalloc(&bitmap, words * sizeof(uint32_t)));
+    QS_HIP(hipMemsetAsync(bitmap, 0, words * sizeof(uint32_t), stream_));
+    dev::math_injective(N, args, d_table, bitmap, d_flags_ + 1, d_flags_, stream_);
+    QS_HIP(hipMemcpyAsync(h_flags_, d_flags_, sizeof(int) * 2, hipMemcpyDeviceToHost, stream_));
+    sync();
+    QS_HIP(hipFree(bitmap));
+    if (h_flags_[0]) throw RuntimeError("emulate_math: a register value does not fit a C++ int");
+    injective = h_flags_[1] == 0;
+  }
+  void *dst = scratch(0);
+  dispatch([&](auto tag) {
+    using R = decltype(tag);
+    if (injective) {
+      dev::math_permute<R>((const amp_t<R> *)psi_, (amp_t<R> *)dst, N, args, d_table, d_flags_, stream_);
+    } else {
+      QS_HIP(hipMemsetAsync(dst, 0, N * amp_bytes(), stream_));
+      dev::math_accumulate<R>((const amp_t<R> *)psi_, (amp_t<R> *)dst, N, args, d_table, d_flags_, stream_);
+    }
+  });
+  QS_HIP(hipMemcpyAsync(h_flags_, d_flags_, sizeof(int), hipMemcpyDeviceToHost, stream_));
+  sync();
+  if (h_flags_[0]) throw RuntimeError("emulate_math: a register value does not fit a C++ int");
+  std::swap(psi_, buf_[0]);
+}
+
+// simulator.hpp:224-290 for the opcodes of include/projectq_amd.h
+void Engine::emulate_math_op(int opcode, const std::vector<int64_t> &params,
+                             const std::vector<std::vector<unsigned>> &regs,
+                             const std::vector<unsigned> &ctrl) {
+  run();
+  for (auto &r : regs) check_known(r, "emulate_math");
+  check_known(ctrl, "emulate_math");
+  dev::MathArgs a = math_args(regs, ctrl);
+  a.opcode = opcode;
+  a.a = params.size() > 0 ? params[0] : 0;
+  a.N = params.size() > 1 ? params[1] : 1;
+  // x -> x + a wraps modulo 2^width on every register independently: a bijection
+  math_apply(a, nullptr, opcode == 0);
+}
+
+// simulator.hpp:224-269 with a host function: evaluated once per distinct tuple of register
+// values into a lookup table, then applied as a device permutation.
+void Engine::emulate_math_callback(const std::function<bool(int32_t *, size_t)> &fn,
+                                   const std::vector<std::vector<unsigned>> &regs,
+                                   const std::vector<unsigned> &ctrl) {
+  run();
+  for (auto &r : regs) check_known(r, "emulate_math");
+  check_known(ctrl, "emulate_math");
+  dev::MathArgs a = math_args(regs, ctrl);
+  a.opcode = -1;
+  unsigned bits = 0;
+  for (auto &r : regs) bits += (unsigned)r.size();
+  if (bits > 30) throw ValueError("emulate_math: a Python function over more than 30 register qubits is not supported");
+  const size_t domain = size_t(1) << bits, nr = regs.size();
+  std::vector<int32_t> table(domain * std::max<size_t>(nr, 1));
+  std::vector<int32_t> v(std::max<size_t>(nr, 1));
+  for (size_t x = 0; x < domain; ++x) {
+    unsigned shift = 0;
+    for (size_t r = 0; r < nr; ++r) {
+      uint64_t w = regs[r].size();
+      v[r] = (int32_t)(uint32_t)((x >> shift) & ((uint64_t(1) << w) - 1));
+      shift += (unsigned)w;
+    }
+    if (!fn(v.data(), nr)) throw CallbackError("emulate_math: the Python function raised");
+    std::copy(v.begin(), v.begin() + nr, table.begin() + x * nr);
+  }
+  int32_t *d_table = nullptr;
+  QS_HIP(hipMalloc(&d_table, table.size() * sizeof(int32_t)));
+  try {
+    QS_HIP(hipMemcpyAsync(d_table, table.data(), table.size() * sizeof(int32_t), hipMemcpyHostToDevice, stream_));
+    math_apply(a, d_table, false);
+  } catch (...) {
+    (void)hipStreamSynchronize(stream_);
+    (void)hipFree(d_table);
+    throw;
+  }
+  QS_HIP(hipFree(d_table));
+}
+
+// ------------------------------------------------------------------------------ Pauli strings
+// A Pauli string applied left to right (simulator.hpp:538-550) equals omega * X^x Z^z:
+// X_q: x ^= q;  Z_q: omega *= (-1)^{x_q}, z ^= q;  Y_q = i X_q Z_q: omega *= i (-1)^{x_q},
+// x ^= q, z ^= q.  (P psi)_i = omega (-1)^{popcount((i^x)&z)} psi_{i^x}.
+void Engine::pauli_encode(const PauliString &t, const std::vector<unsigned> &ids, uint64_t *x,
+                          uint64_t *z, cplx *omega) const {
+  uint64_t X = 0, Z = 0;
+  cplx w(1.0, 0.0);
+  for (auto &op : t.ops) {
+    if (op.first >= ids.size()) throw ValueError("qubit_operator acts on more qubits than contained in the qureg.");
+    const uint64_t q = uint64_t(1) << pos_.at(ids[op.first]);
+    const double sgn = (X & q) ? -1.0 : 1.0;
+    switch (op.second) {
+      case 'X': X ^= q; break;
+      case 'Z': w *= sgn; Z ^= q; break;
+      case 'Y': w *= cplx(0.0, sgn); X ^= q; Z ^= q; break;
+      default: throw ValueError("Pauli operator must be X, Y or Z");
+    }
+  }
+  *x = X;
+  *z = Z;
+  *omega = w;
+}
+
+// simulator.hpp:292-322: sum_t c_t Re<psi|P_t|psi>, one read-only pass per string.
+double Engine::get_expectation_value(const std::vector<PauliString> &terms,
+                                     const std::vector<double> &coeffs,
+                                     const std::vector<unsigned> &ids) {
+  run();
+  check_known(ids, "get_expectation_value");
+  double total = 0.0;
+  for (size_t t = 0; t < terms.size(); ++t) {
+    uint64_t x, z;
+    cplx w;
+    pauli_encode(terms[t], ids, &x, &z, &w);
+    double delta;
+    if (x == 0 && z == 0) {
+      delta = w.real() * masked_norm(0, 0);
+    } else {
+      dispatch([&](auto tag) {
+        using R = decltype(tag);
+        dev::pauli_expect<R>((const amp_t<R> *)psi_, amps(), x, z, d_partials_, stream_);
+      });
+      double si;
+      double sr = finish(dev::kReduceGrid, 2, &si);
+      delta = (w * cplx(sr, si)).real();
+    }
+    total += coeffs[t] * delta;
+  }
+  return total;
+}
+
+// simulator.hpp:324-350: psi <- sum_t c_t P_t psi in one out-of-place pass.
+void Engine::apply_qubit_operator(const std::vector<PauliString> &terms, const std::vector<cplx> &coeffs,
+                                  const std::vector<unsigned> &ids) {
+  run();
+  check_known(ids, "apply_qubit_operator");
+  std::vector<dev::PauliTerm> pt(terms.size());
+  for (size_t t = 0; t < terms.size(); ++t) {
+    cplx w;
+    pauli_encode(terms[t], ids, &pt[t].x, &pt[t].z, &w);
+    cplx c = coeffs[t] * w;
+    pt[t].c_re = c.real();
+    pt[t].c_im = c.imag();
+  }
+  const dev::PauliTerm *d_terms =
+      pt.empty() ? nullptr : (const dev::PauliTerm *)upload(pt.data(), pt.size() * sizeof(dev::PauliTerm));
+  void *dst = scratch(0);
+  dispatch([&](auto tag) {
+    using R = decltype(tag);
+    dev::pauli_sum<R>((const amp_t<R> *)psi_, (amp_t<R> *)dst, nullptr, amps(), d_terms, (int)pt.size(), 1.0,
+                      0.0, false, 0, d_partials_, stream_);
+  });
+  std::swap(psi_, buf_[0]);
+}
+
+// simulator.hpp:386-438: scaled, truncated Taylor series of exp(-i H t) on the controlled
+// subspace; each Taylor term is one pass that also accumulates the output and the update norm.
+void Engine::emulate_time_evolution(const std::vector<PauliString> &terms, const std::vector<double> &coeffs,
+                                    double time, const std::vector<unsigned> &ids,
+                                    const std::vector<unsigned> &ctrl) {
+  run();
+  check_known(ids, "emulate_time_evolution");
+  check_known(ctrl, "emulate_time_evolution");
+  const cplx I(0.0, 1.0);
+  double tr = 0.0, op_nrm = 0.0;
+  std::vector<dev::PauliTerm> pt;
+  for (size_t t = 0; t < terms.size(); ++t) {
+    if (terms[t].ops.empty()) {
+      tr += coeffs[t];
+      continue;
+    }
+    dev::PauliTerm p;
+    cplx w;
+    pauli_encode(terms[t], ids, &p.x, &p.z, &w);
+    cplx c = coeffs[t] * w;
+    p.c_re = c.real();
+    p.c_im = c.imag();
+    pt.push_back(p);
+    op_nrm += std::abs(coeffs[t]);
+  }
+  const unsigned steps = (unsigned)(std::abs(time) * op_nrm + 1.);
+  const cplx correction = std::exp(-time * I * tr / (double)steps);
+  const uint64_t cmask = mask_of(ctrl), N = amps();
+  const dev::PauliTerm *d_terms =
+      pt.empty() ? nullptr : (const dev::PauliTerm *)upload(pt.data(), pt.size() * sizeof(dev::PauliTerm));
+  void *out = scratch(0);
+  scratch(1);  // buf_[1] ping-pongs with psi_ as the Taylor term; psi_, buf_[0], buf_[1] stay distinct
+  QS_HIP(hipMemcpyAsync(out, psi_, N * amp_bytes(), hipMemcpyDeviceToDevice, stream_));
+  for (unsigned s = 0; s < steps; ++s) {
+    double change = 1.0;
+    for (unsigned k = 0; change > 1.e-12; ++k) {
+      const cplx f = (-time * I) / double(steps * (k + 1));
+      dispatch([&](auto tag) {
+        using R = decltype(tag);
+        dev::pauli_sum<R>((const amp_t<R> *)psi_, (amp_t<R> *)buf_[1], (amp_t<R> *)out, N, d_terms,
+                          (int)pt.size(), f.real(), f.imag(), true, cmask, d_partials_, stream_);
+      });
+      change = std::sqrt(finish(dev::kReduceGrid, 1));
+      std::swap(psi_, buf_[1]);
+    }
+    dispatch([&](auto tag) {
+      using R = decltype(tag);
+      dev::scale_masked<R>((amp_t<R> *)out, N, cmask, correction.real(), correction.imag(), stream_);
+    });
+    QS_HIP(hipMemcpyAsync(psi_, out, N * amp_bytes(), hipMemcpyDeviceToDevice, stream_));
+  }
+  sync();
+}
+
+// ------------------------------------------------------------------------------ queries
+// simulator.hpp:352-368
+double Engine::get_probability(const std::vector<uint8_t> &bits, const std::vector<unsigned> &ids) {
+  run();
+  for (unsigned q : ids)
+    if (!pos_.count(q))
+      throw RuntimeError("get_probability(): Unknown qubit id. Please make sure you have called eng.flush().");
+  uint64_t mask = 0, val = 0;
+  for (size_t i = 0; i < ids.size(); ++i) {
+    mask |= uint64_t(1) << pos_[ids[i]];
+    val |= uint64_t(bits[i] ? 1 : 0) << pos_[ids[i]];
+  }
+  return masked_norm(mask, val & mask);
+}
+
+// simulator.hpp:370-384
+cplx Engine::get_amplitude(const std::vector<uint8_t> &bits, const std::vector<unsigned> &ids) {
+  run();
+  uint64_t chk = 0, index = 0;
+  for (size_t i = 0; i < ids.size(); ++i) {
+    auto it = pos_.find(ids[i]);
+    if (it == pos_.end()) break;
+    chk |= uint64_t(1) << it->second;
+    index |= uint64_t(bits[i] ? 1 : 0) << it->second;
+  }
+  if (chk + 1 != amps())
+    throw RuntimeError(
+        "The second argument to get_amplitude() must be a permutation of all allocated qubits. Please make "
+        "sure you have called eng.flush().");
+  if (c64_) {
+    float a[2];
+    QS_HIP(hipMemcpyAsync(a, (char *)psi_ + index * 8, 8, hipMemcpyDeviceToHost, stream_));
+    sync();
+    return cplx(a[0], a[1]);
+  }
+  double a[2];
+  QS_HIP(hipMemcpyAsync(a, (char *)psi_ + index * 16, 16, hipMemcpyDeviceToHost, stream_));
+  sync();
+  return cplx(a[0], a[1]);
+}
+
+// simulator.hpp:440-454
+void Engine::set_wavefunction(const double *wf, size_t n_amps, const std::vector<unsigned> &ordering) {
+  run();
+  bool ok = pos_.size() == ordering.size();
+  for (unsigned q : ordering) ok = ok && pos_.count(q);
+  if (!ok)
+    throw RuntimeError(
+        "set_wavefunction(): Invalid mapping provided. Please make sure all qubits have been allocated "
+        "previously (call eng.flush()).");
+  if (n_amps != (uint64_t(1) << ordering.size()))
+    throw ValueError("set_wavefunction(): the wavefunction must hold 2^len(ordering) amplitudes");
+  for (size_t i = 0; i < ordering.size(); ++i) pos_[ordering[i]] = (unsigned)i;
+  if (c64_) {
+    std::vector<float> f(2 * n_amps);
+    for (size_t i = 0; i < 2 * n_amps; ++i) f[i] = (float)wf[i];
+    QS_HIP(hipMemcpyAsync(psi_, f.data(), f.size() * sizeof(float), hipMemcpyHostToDevice, stream_));
+    sync();
+  } else {
+    QS_HIP(hipMemcpyAsync(psi_, wf, n_amps * 16, hipMemcpyHostToDevice, stream_));
+    sync();
+  }
+}
+
+// simulator.hpp:456-485
+void Engine::collapse_wavefunction(const std::vector<unsigned> &ids, const std::vector<uint8_t> &values) {
+  run();
+  if (ids.size() != values.size()) throw ValueError("collapse_wavefunction(): ids and values size mismatch");
+  for (unsigned q : ids)
+    if (!pos_.count(q))
+      throw RuntimeError(
+          "collapse_wavefunction(): Unknown qubit id(s) provided. Try calling eng.flush() before invoking "
+          "this function.");
+  uint64_t mask = 0, val = 0;
+  for (size_t i = 0; i < ids.size(); ++i) {
+    mask |= uint64_t(1) << pos_[ids[i]];
+    val |= uint64_t(values[i] ? 1 : 0) << pos_[ids[i]];
+  }
+  val &= mask;
+  const double p = masked_norm(mask, val);
+  if (p < 1.e-12) throw RuntimeError("collapse_wavefunction(): Invalid collapse! Probability is ~0.");
+  const double scale = 1. / std::sqrt(p);
+  dispatch([&](auto tag) {
+    using R = decltype(tag);
+    dev::collapse<R>((amp_t<R> *)psi_, amps(), mask, val, scale, stream_);
+  });
+}
+
+// simulator.hpp:529-532
+void Engine::cheat_sizes(size_t *n_qubits, size_t *n_amps) {
+  run();
+  sync();
+  *n_qubits = pos_.size();
+  *n_amps = amps();
+}
+
+void Engine::cheat(uint32_t *ids, uint32_t *positions, double *state) {
+  run();
+  size_t k = 0;
+  for (auto &kv : pos_) {
+    ids[k] = kv.first;
+    positions[k] = kv.second;
+    ++k;
+  }
+  const uint64_t N = amps();
+  if (c64_) {
+    std::vector<float> f(2 * N);
+    QS_HIP(hipMemcpyAsync(f.data(), psi_, N * 8, hipMemcpyDeviceToHost, stream_));
+    sync();
+    for (size_t i = 0; i < 2 * N; ++i) state[i] = f[i];
+  } else {
+    QS_HIP(hipMemcpyAsync(state, psi_, N * 16, hipMemcpyDeviceToHost, stream_));
+    sync();
+  }
+}
+
+}  // namespace qs

@@ -1,0 +1,115 @@
+// The simulator engine: owns the resident HBM state, the qubit map, the RNG and the gate queue,
+// and implements every method of the reference's class Simulator
+// (projectq/backends/_sim/_cppkernels/simulator.hpp:37-579) on the GPU.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <functional>
+#include <map>
+#include <random>
+#include <utility>
+#include <vector>
+
+#include "common.hpp"
+#include "fusion.hpp"
+#include "kernels.hpp"
+
+namespace qs {
+
+struct PauliString {
+  std::vector<std::pair<uint32_t, char>> ops;  // (local index into ids, 'X'|'Y'|'Z')
+};
+
+class Engine {
+ public:
+  Engine(uint32_t seed, int precision, int fusion_mode, int planner, unsigned capacity_hint);
+  ~Engine();
+  Engine(const Engine &) = delete;
+  Engine &operator=(const Engine &) = delete;
+
+  void allocate_qubit(unsigned id);
+  void deallocate_qubit(unsigned id);
+  bool get_classical_value(unsigned id, double tol);
+  bool is_classical(unsigned id, double tol);
+  std::vector<uint8_t> measure_qubits(const std::vector<unsigned> &ids);
+  void apply_controlled_gate(Gate g);
+  void emulate_math_op(int opcode, const std::vector<int64_t> &params,
+                       const std::vector<std::vector<unsigned>> &regs,
+                       const std::vector<unsigned> &ctrl);
+  void emulate_math_callback(const std::function<bool(int32_t *, size_t)> &fn,
+                             const std::vector<std::vector<unsigned>> &regs,
+                             const std::vector<unsigned> &ctrl);
+  double get_expectation_value(const std::vector<PauliString> &terms,
+                               const std::vector<double> &coeffs,
+                               const std::vector<unsigned> &ids);
+  void apply_qubit_operator(const std::vector<PauliString> &terms, const std::vector<cplx> &coeffs,
+                            const std::vector<unsigned> &ids);
+  void emulate_time_evolution(const std::vector<PauliString> &terms,
+                              const std::vector<double> &coeffs, double time,
+                              const std::vector<unsigned> &ids, const std::vector<unsigned> &ctrl);
+  double get_probability(const std::vector<uint8_t> &bits, const std::vector<unsigned> &ids);
+  cplx get_amplitude(const std::vector<uint8_t> &bits, const std::vector<unsigned> &ids);
+  void set_wavefunction(const double *wf, size_t n_amps, const std::vector<unsigned> &ordering);
+  void collapse_wavefunction(const std::vector<unsigned> &ids, const std::vector<uint8_t> &values);
+  void run();
+  void cheat_sizes(size_t *n_qubits, size_t *n_amps);
+  void cheat(uint32_t *ids, uint32_t *positions, double *state);
+
+  // execute one fused block (also used by the planners)
+  void exec_block(const Block &b);
+  unsigned num_qubits() const { return n_; }
+  unsigned position(unsigned id) const { return pos_.at(id); }
+  hipStream_t stream() const { return stream_; }
+  bool is_c64() const { return c64_; }
+
+ private:
+  template <class F>
+  void dispatch(F &&f) {
+    if (c64_) f(float{});
+    else f(double{});
+  }
+  size_t amp_bytes() const { return c64_ ? 8 : 16; }
+  uint64_t amps() const { return uint64_t(1) << n_; }
+  void ensure_capacity(uint64_t need_amps);
+  void *scratch(int which);  // full-capacity scratch buffer 0 or 1 (lazy)
+  void release_scratch();
+  const void *upload(const void *host, size_t bytes);
+  double finish(int n_partials, int width, double *second = nullptr);
+  void sync();
+  void check_known(const std::vector<unsigned> &ids, const char *what) const;
+  uint64_t mask_of(const std::vector<unsigned> &ids) const;
+  void pauli_encode(const PauliString &t, const std::vector<unsigned> &ids, uint64_t *x,
+                    uint64_t *z, cplx *omega) const;
+  dev::MathArgs math_args(const std::vector<std::vector<unsigned>> &regs,
+                          const std::vector<unsigned> &ctrl) const;
+  void math_apply(dev::MathArgs args, const int32_t *d_table, bool known_bijective);
+  double masked_norm(uint64_t mask, uint64_t val);
+  void flush_window();
+
+  bool c64_;
+  int fusion_mode_, planner_kind_;
+  int device_ = 0;
+  unsigned n_ = 0;
+  unsigned cap_hint_ = 0;
+  uint64_t cap_ = 0;  // amplitudes the buffers hold
+  void *psi_ = nullptr;
+  void *buf_[2] = {nullptr, nullptr};
+  std::map<unsigned, unsigned> pos_;
+  GateQueue queue_;
+  std::vector<Gate> window_;
+  std::mt19937 rng_;
+  std::uniform_real_distribution<double> uni_{0.0, 1.0};
+  hipStream_t stream_ = nullptr;
+  double *d_partials_ = nullptr, *d_result_ = nullptr, *h_result_ = nullptr;
+  int *d_flags_ = nullptr, *h_flags_ = nullptr;
+  unsigned long long *d_key_ = nullptr, *h_key_ = nullptr;
+  double *d_chunks_ = nullptr, *h_chunks_ = nullptr;
+  size_t chunks_cap_ = 0;
+  char *d_arena_ = nullptr, *h_arena_ = nullptr;
+  size_t arena_off_ = 0;
+  static constexpr size_t kArenaBytes = size_t(8) << 20;
+};
+
+}  // namespace qs

@@ -1,0 +1,707 @@
+// HIP kernels of the MI355X state-vector simulator (gfx950, wave64).
+//
+// Every kernel here is HBM-bound: one streaming pass over (a subspace of) the resident state.
+// The dense-block kernel keeps the 2^k x 2^k matrix in scalar registers (uniform loads), holds
+// the 2^k amplitudes of one group in VGPRs and streams each amplitude once in and once out.
+// Reductions accumulate in FP64 whatever the state precision, with a fixed grid and a fixed
+// final order so results are deterministic run to run.
+
+#include <hip/hip_runtime.h>
+#include <hipcub/hipcub.hpp>
+
+#include <algorithm>
+#include <type_traits>
+
+#include "kernels.hpp"
+
+namespace qs {
+namespace dev {
+namespace {
+
+template <typename A>
+__device__ __forceinline__ double re(const A &a) {
+  return (double)a.x;
+}
+template <typename A>
+__device__ __forceinline__ double im(const A &a) {
+  return (double)a.y;
+}
+template <typename R>
+__device__ __forceinline__ amp_t<R> make_amp(double x, double y) {
+  amp_t<R> a;
+  a.x = (R)x;
+  a.y = (R)y;
+  return a;
+}
+
+__device__ __forceinline__ uint64_t insert_zeros(uint64_t g, const BitInsert &ins) {
+  for (uint32_t i = 0; i < ins.count; ++i) {
+    uint32_t p = ins.pos[i];
+    g = ((g >> p) << (p + 1)) | (g & ((uint64_t(1) << p) - 1));
+  }
+  return g;
+}
+
+__device__ __forceinline__ uint64_t insert_one_zero(uint64_t g, int p) {
+  return ((g >> p) << (p + 1)) | (g & ((uint64_t(1) << p) - 1));
+}
+
+inline int grid_for(uint64_t work, int cap = 16384) {
+  uint64_t g = (work + kBlock - 1) / kBlock;
+  if (g < 1) g = 1;
+  if (g > (uint64_t)cap) g = cap;
+  return (int)g;
+}
+
+// Sum over a workgroup of kBlock threads; result valid in thread 0.
+__device__ __forceinline__ double block_sum(double v, double *lds) {
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+  int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  if (lane == 0) lds[wave] = v;
+  __syncthreads();
+  double t = 0.0;
+  if (threadIdx.x == 0)
+    for (int w = 0; w < kBlock / 64; ++w) t += lds[w];
+  __syncthreads();
+  return t;
+}
+
+// Neumaier-compensated accumulator (used by the expectation value).
+struct Kahan {
+  double s = 0.0, c = 0.0;
+  __device__ __forceinline__ void add(double v) {
+    double t = s + v;
+    if (fabs(s) >= fabs(v)) c += (s - t) + v;
+    else c += (v - t) + s;
+    s = t;
+  }
+  __device__ __forceinline__ double value() const { return s + c; }
+};
+
+// ------------------------------------------------------------------------------ gate blocks
+template <int K>
+struct Offsets {
+  uint64_t o[1 << K];
+};
+
+template <typename R, int K>
+__global__ __launch_bounds__(kBlock) void k_apply(amp_t<R> *__restrict__ psi, uint64_t n_groups,
+                                                  BitInsert ins, uint64_t cmask, Offsets<K> off,
+                                                  const amp_t<R> *__restrict__ m) {
+  constexpr int D = 1 << K;
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; g < n_groups; g += stride) {
+    const uint64_t base = insert_zeros(g, ins) | cmask;
+    R vr[D], vi[D];
+#pragma unroll
+    for (int j = 0; j < D; ++j) {
+      amp_t<R> a = psi[base + off.o[j]];
+      vr[j] = a.x;
+      vi[j] = a.y;
+    }
+#pragma unroll(K <= 3 ? D : 1)
+    for (int r = 0; r < D; ++r) {
+      R ar = 0, ai = 0;
+#pragma unroll
+      for (int c = 0; c < D; ++c) {
+        const amp_t<R> mc = m[r * D + c];
+        ar = fma(mc.x, vr[c], ar);
+        ar = fma(-mc.y, vi[c], ar);
+        ai = fma(mc.x, vi[c], ai);
+        ai = fma(mc.y, vr[c], ai);
+      }
+      amp_t<R> o;
+      o.x = ar;
+      o.y = ai;
+      psi[base + off.o[r]] = o;
+    }
+  }
+}
+
+template <typename R, int K>
+void launch_apply(amp_t<R> *psi, uint64_t n_groups, const BitInsert &ins, uint64_t cmask,
+                  const uint32_t *tpos, const amp_t<R> *m, hipStream_t s) {
+  Offsets<K> off;
+  for (int j = 0; j < (1 << K); ++j) {
+    uint64_t o = 0;
+    for (int l = 0; l < K; ++l)
+      if ((j >> l) & 1) o |= uint64_t(1) << tpos[l];
+    off.o[j] = o;
+  }
+  hipLaunchKernelGGL((k_apply<R, K>), dim3(grid_for(n_groups)), dim3(kBlock), 0, s, psi, n_groups,
+                     ins, cmask, off, m);
+  QS_HIP(hipGetLastError());
+}
+
+// ------------------------------------------------------------------------------ reductions
+template <typename R>
+__global__ __launch_bounds__(kBlock) void k_norm_masked(const amp_t<R> *__restrict__ psi,
+                                                        uint64_t n_sub, BitInsert ins,
+                                                        uint64_t val, double *partials) {
+  __shared__ double lds[kBlock / 64];
+  double acc = 0.0;
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; g < n_sub; g += stride) {
+    amp_t<R> a = psi[insert_zeros(g, ins) | val];
+    double x = re(a), y = im(a);
+    acc = fma(x, x, fma(y, y, acc));
+  }
+  double t = block_sum(acc, lds);
+  if (threadIdx.x == 0) partials[blockIdx.x] = t;
+}
+
+__global__ void k_finish_sum(const double *__restrict__ partials, int n_partials, int width,
+                             double *out) {
+  int j = threadIdx.x;
+  if (j >= width) return;
+  double t = 0.0;
+  for (int b = 0; b < n_partials; ++b) t += partials[(size_t)b * width + j];
+  out[j] = t;
+}
+
+template <typename R>
+__global__ __launch_bounds__(kBlock) void k_collapse(amp_t<R> *__restrict__ psi, uint64_t n,
+                                                     uint64_t mask, uint64_t val, double scale) {
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+    amp_t<R> a = psi[i];
+    if ((i & mask) != val) a = make_amp<R>(0.0, 0.0);
+    else a = make_amp<R>(re(a) * scale, im(a) * scale);
+    psi[i] = a;
+  }
+}
+
+template <typename R>
+__global__ __launch_bounds__(kBlock) void k_chunk_norms(const amp_t<R> *__restrict__ psi,
+                                                        int chunk_log2, double *sums) {
+  __shared__ double lds[kBlock / 64];
+  const uint64_t base = (uint64_t)blockIdx.x << chunk_log2;
+  const uint64_t len = uint64_t(1) << chunk_log2;
+  double acc = 0.0;
+  for (uint64_t j = threadIdx.x; j < len; j += blockDim.x) {
+    amp_t<R> a = psi[base + j];
+    double x = re(a), y = im(a);
+    acc = fma(x, x, fma(y, y, acc));
+  }
+  double t = block_sum(acc, lds);
+  if (threadIdx.x == 0) sums[blockIdx.x] = t;
+}
+
+// One workgroup: thread t owns the contiguous slice [t*per, (t+1)*per) of the chunk.
+template <typename R>
+__global__ __launch_bounds__(kBlock) void k_chunk_pick(const amp_t<R> *__restrict__ psi,
+                                                       uint64_t start, uint64_t len, double base,
+                                                       double rnd, unsigned long long *pick) {
+  __shared__ double part[kBlock];
+  const uint64_t per = (len + kBlock - 1) / kBlock;
+  const uint64_t lo = min<uint64_t>(len, threadIdx.x * per), hi = min<uint64_t>(len, lo + per);
+  double mine = 0.0;
+  for (uint64_t j = lo; j < hi; ++j) {
+    amp_t<R> a = psi[start + j];
+    double x = re(a), y = im(a);
+    mine += x * x + y * y;
+  }
+  part[threadIdx.x] = mine;
+  __syncthreads();
+  if (threadIdx.x == 0) {  // exclusive prefix in slice order
+    double run = base;
+    for (int t = 0; t < kBlock; ++t) {
+      double v = part[t];
+      part[t] = run;
+      run += v;
+    }
+  }
+  __syncthreads();
+  double run = part[threadIdx.x];
+  for (uint64_t j = lo; j < hi; ++j) {
+    amp_t<R> a = psi[start + j];
+    double x = re(a), y = im(a);
+    run += x * x + y * y;
+    if (run >= rnd) {
+      atomicMin(pick, (unsigned long long)(start + j));
+      break;
+    }
+  }
+}
+
+template <typename R>
+__global__ __launch_bounds__(kBlock) void k_classical_flags(const amp_t<R> *__restrict__ psi,
+                                                            uint64_t n, int p, double tol,
+                                                            int *flags) {
+  int lo = 0, hi = 0;
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+    amp_t<R> a = psi[i];
+    double x = re(a), y = im(a);
+    int big = (x * x + y * y) > tol;
+    if ((i >> p) & 1) hi |= big;
+    else lo |= big;
+  }
+  lo = __syncthreads_or(lo);
+  hi = __syncthreads_or(hi);
+  if (threadIdx.x == 0) {
+    if (lo) atomicOr(&flags[0], 1);
+    if (hi) atomicOr(&flags[1], 1);
+  }
+}
+
+template <typename R>
+__global__ __launch_bounds__(kBlock) void k_first_pair(const amp_t<R> *__restrict__ psi,
+                                                       uint64_t n_pairs, int p, double tol,
+                                                       unsigned long long *key) {
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  unsigned long long best = ~0ull;
+  for (uint64_t q = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; q < n_pairs; q += stride) {
+    const uint64_t i0 = insert_one_zero(q, p), i1 = i0 | (uint64_t(1) << p);
+    amp_t<R> a = psi[i0], b = psi[i1];
+    double n0 = re(a) * re(a) + im(a) * im(a), n1 = re(b) * re(b) + im(b) * im(b);
+    if (n0 > tol) {
+      best = min(best, (unsigned long long)(2 * q));
+      break;  // q only grows in this thread
+    }
+    if (n1 > tol) {
+      best = min(best, (unsigned long long)(2 * q + 1));
+      break;
+    }
+  }
+  if (best != ~0ull) atomicMin(key, best);
+}
+
+template <typename R>
+__global__ __launch_bounds__(kBlock) void k_drop_bit(const amp_t<R> *__restrict__ src,
+                                                     amp_t<R> *__restrict__ dst, uint64_t n_out,
+                                                     int p, int value) {
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  const uint64_t set = uint64_t(value ? 1 : 0) << p;
+  for (uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; j < n_out; j += stride)
+    dst[j] = src[insert_one_zero(j, p) | set];
+}
+
+// ------------------------------------------------------------------------------ Pauli strings
+template <typename R>
+__global__ __launch_bounds__(kBlock) void k_pauli_expect(const amp_t<R> *__restrict__ psi,
+                                                         uint64_t n, uint64_t x, uint64_t z,
+                                                         double *partials) {
+  __shared__ double lds[kBlock / 64];
+  Kahan sr, si;
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+    const uint64_t j = i ^ x;
+    amp_t<R> a = psi[i], b = psi[j];
+    double s = (__popcll(j & z) & 1) ? -1.0 : 1.0;
+    sr.add(s * (re(a) * re(b) + im(a) * im(b)));
+    si.add(s * (re(a) * im(b) - im(a) * re(b)));
+  }
+  double tr = block_sum(sr.value(), lds);
+  double ti = block_sum(si.value(), lds);
+  if (threadIdx.x == 0) {
+    partials[2 * blockIdx.x] = tr;
+    partials[2 * blockIdx.x + 1] = ti;
+  }
+}
+
+template <typename R, bool EVOLVE>
+__global__ __launch_bounds__(kBlock) void k_pauli_sum(const amp_t<R> *__restrict__ src,
+                                                      amp_t<R> *__restrict__ dst,
+                                                      amp_t<R> *__restrict__ out, uint64_t n,
+                                                      const PauliTerm *__restrict__ terms,
+                                                      int n_terms, double f_re, double f_im,
+                                                      uint64_t cmask, double *partials) {
+  __shared__ double lds[kBlock / 64];
+  double nrm = 0.0;
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+    double ur = 0.0, ui = 0.0;
+    for (int t = 0; t < n_terms; ++t) {
+      const PauliTerm pt = terms[t];
+      const uint64_t j = i ^ pt.x;
+      amp_t<R> b = src[j];
+      double br = re(b), bi = im(b);
+      if (__popcll(j & pt.z) & 1) {
+        br = -br;
+        bi = -bi;
+      }
+      ur += br * pt.c_re - bi * pt.c_im;
+      ui += br * pt.c_im + bi * pt.c_re;
+    }
+    if (EVOLVE) {
+      double vr = ur * f_re - ui * f_im, vi = ur * f_im + ui * f_re;
+      ur = vr;
+      ui = vi;
+    }
+    dst[i] = make_amp<R>(ur, ui);
+    if (EVOLVE && (i & cmask) == cmask) {
+      amp_t<R> o = out[i];
+      out[i] = make_amp<R>(re(o) + ur, im(o) + ui);
+      nrm += ur * ur + ui * ui;
+    }
+  }
+  if (EVOLVE) {
+    double t = block_sum(nrm, lds);
+    if (threadIdx.x == 0) partials[blockIdx.x] = t;
+  }
+}
+
+template <typename R>
+__global__ __launch_bounds__(kBlock) void k_scale_masked(amp_t<R> *__restrict__ psi, uint64_t n,
+                                                         uint64_t cmask, double cr, double ci) {
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+    if ((i & cmask) != cmask) continue;
+    amp_t<R> a = psi[i];
+    psi[i] = make_amp<R>(re(a) * cr - im(a) * ci, re(a) * ci + im(a) * cr);
+  }
+}
+
+// ------------------------------------------------------------------------------ emulate_math
+__device__ __forceinline__ int32_t wrap32(int64_t x) { return (int32_t)(uint32_t)(uint64_t)x; }
+__device__ __forceinline__ int32_t check32(int64_t x, int &bad) {
+  if (x < INT32_MIN || x > INT32_MAX) bad = 1;
+  return (int32_t)x;
+}
+__device__ __forceinline__ int64_t floordiv(int64_t a, int64_t b) {
+  int64_t q = a / b;
+  if ((a % b != 0) && ((a < 0) != (b < 0))) --q;
+  return q;
+}
+__device__ __forceinline__ int64_t py_bitlen(int64_t x) {  // len(bin(x)[2:])
+  if (x == 0) return 1;
+  if (x > 0) return 64 - __clzll((unsigned long long)x);
+  return 64 - __clzll((unsigned long long)(-x)) + 1;
+}
+
+// new values of the registers for opcode args.opcode (include/projectq_amd.h QS_MATH_*)
+__device__ void math_eval(const MathArgs &A, int32_t *v, int &bad) {
+  switch (A.opcode) {
+    case 0:
+      for (int r = 0; r < A.n_regs; ++r) v[r] = wrap32((int64_t)v[r] + (int32_t)A.a);
+      break;
+    case 1:
+      for (int r = 0; r < A.n_regs; ++r)
+        v[r] = wrap32(wrap32((int64_t)v[r] + (int32_t)A.a) % (int32_t)A.N);
+      break;
+    case 2:
+      for (int r = 0; r < A.n_regs; ++r)
+        v[r] = wrap32(wrap32((int64_t)v[r] * (int32_t)A.a) % (int32_t)A.N);
+      break;
+    case 3: {
+      int64_t s = (int64_t)v[0] + v[1];
+      if (py_bitlen(s) > (int64_t)A.width[0]) {
+        int64_t mod = int64_t(1) << A.width[0];
+        s = ((s % mod) + mod) % mod;
+        if (A.n_regs == 3) v[2] ^= 1;
+      }
+      v[1] = check32(s, bad);
+      break;
+    }
+    case 4:
+      v[1] = check32((int64_t)v[1] - v[0], bad);
+      break;
+    case 5:
+      if (A.n_regs == 3) v[2] ^= 1;
+      v[1] = check32((int64_t)v[1] - v[0], bad);
+      break;
+    case 6:
+      if (v[1] < v[0]) v[2] = (v[2] == 0) ? 1 : 0;
+      break;
+    case 7:
+      v[2] = check32((int64_t)v[2] + (int64_t)v[0] * v[1], bad);
+      break;
+    case 8:
+      v[2] = check32((int64_t)v[2] - (int64_t)v[0] * v[1], bad);
+      break;
+    case 9: {
+      int64_t dividend = v[0], rem = v[1], divisor = v[2];
+      if (divisor == 0 || divisor > dividend) {
+        v[0] = check32(rem, bad);
+        v[1] = check32(dividend, bad);
+      } else {
+        int64_t q = rem + floordiv(dividend, divisor);
+        v[0] = check32(dividend - q * divisor, bad);
+        v[1] = check32(q, bad);
+      }
+      break;
+    }
+    case 10: {
+      int64_t rem = v[0], quo = v[1], divisor = v[2];
+      if (divisor == 0) {
+        v[0] = check32(quo, bad);
+        v[1] = check32(rem, bad);
+      } else {
+        v[0] = check32(rem + quo * divisor, bad);
+        v[1] = 0;
+      }
+      break;
+    }
+    default:
+      bad = 1;
+  }
+}
+
+// destination index of basis state i (simulator.hpp:246-265)
+__device__ uint64_t math_dest(uint64_t i, const MathArgs &A, const int32_t *table, int &bad) {
+  if ((i & A.cmask) != A.cmask) return i;
+  int32_t v[kMaxRegs];
+  uint64_t packed = 0;
+  int shift = 0;
+  for (int r = 0; r < A.n_regs; ++r) {
+    uint64_t x = 0;
+    for (int b = 0; b < A.width[r]; ++b) x |= ((i >> A.pos[r][b]) & 1ull) << b;
+    v[r] = (int32_t)(uint32_t)x;
+    packed |= x << shift;
+    shift += A.width[r];
+  }
+  if (table) {
+    for (int r = 0; r < A.n_regs; ++r) v[r] = table[packed * A.n_regs + r];
+  } else {
+    math_eval(A, v, bad);
+  }
+  uint64_t j = i;
+  for (int r = 0; r < A.n_regs; ++r)
+    for (int b = 0; b < A.width[r]; ++b) {
+      uint64_t want = (uint64_t)(((int64_t)v[r] >> (b < 63 ? b : 63)) & 1);
+      j = (j & ~(uint64_t(1) << A.pos[r][b])) | (want << A.pos[r][b]);
+    }
+  return j;
+}
+
+template <typename R>
+__global__ __launch_bounds__(kBlock) void k_math_permute(const amp_t<R> *__restrict__ src,
+                                                         amp_t<R> *__restrict__ dst, uint64_t n,
+                                                         MathArgs A, const int32_t *table,
+                                                         int *err) {
+  int bad = 0;
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+    uint64_t j = math_dest(i, A, table, bad);
+    amp_t<R> a = src[i];
+    dst[j] = make_amp<R>(0.0 + re(a), 0.0 + im(a));  // newvec starts at 0 (simulator.hpp:239-240)
+  }
+  if (bad) atomicOr(err, 1);
+}
+
+__global__ __launch_bounds__(kBlock) void k_math_injective(uint64_t n, MathArgs A,
+                                                           const int32_t *table, uint32_t *bitmap,
+                                                           int *collide, int *err) {
+  int bad = 0, hit = 0;
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+    uint64_t j = math_dest(i, A, table, bad);
+    uint32_t bit = 1u << (j & 31);
+    uint32_t old = atomicOr(&bitmap[j >> 5], bit);
+    if (old & bit) hit = 1;
+  }
+  if (bad) atomicOr(err, 1);
+  if (hit) atomicOr(collide, 1);
+}
+
+__global__ __launch_bounds__(kBlock) void k_math_keys(uint64_t n, MathArgs A, const int32_t *table,
+                                                      uint32_t *keys, uint32_t *vals, int *err) {
+  int bad = 0;
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+    keys[i] = (uint32_t)math_dest(i, A, table, bad);
+    vals[i] = (uint32_t)i;
+  }
+  if (bad) atomicOr(err, 1);
+}
+
+template <typename R>
+__global__ __launch_bounds__(kBlock) void k_math_segments(const amp_t<R> *__restrict__ src,
+                                                          amp_t<R> *__restrict__ dst, uint64_t n,
+                                                          const uint32_t *keys,
+                                                          const uint32_t *vals) {
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; j < n; j += stride) {
+    if (j > 0 && keys[j] == keys[j - 1]) continue;
+    double sr = 0.0, si = 0.0;
+    uint64_t k = j;
+    do {
+      amp_t<R> a = src[vals[k]];
+      sr += re(a);
+      si += im(a);
+      ++k;
+    } while (k < n && keys[k] == keys[j]);
+    dst[keys[j]] = make_amp<R>(sr, si);
+  }
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------------------ launchers
+template <typename R>
+void apply_block(amp_t<R> *psi, uint64_t n_groups, const BitInsert &ins, uint64_t cmask,
+                 const uint32_t *tpos, int k, const amp_t<R> *d_mat, hipStream_t s) {
+  switch (k) {
+    case 1: launch_apply<R, 1>(psi, n_groups, ins, cmask, tpos, d_mat, s); break;
+    case 2: launch_apply<R, 2>(psi, n_groups, ins, cmask, tpos, d_mat, s); break;
+    case 3: launch_apply<R, 3>(psi, n_groups, ins, cmask, tpos, d_mat, s); break;
+    case 4: launch_apply<R, 4>(psi, n_groups, ins, cmask, tpos, d_mat, s); break;
+    case 5: launch_apply<R, 5>(psi, n_groups, ins, cmask, tpos, d_mat, s); break;
+    default: throw ValueError("Gates with more than 5 qubits are not supported!");
+  }
+}
+
+template <typename R>
+void norm_masked(const amp_t<R> *psi, uint64_t n_sub, const BitInsert &ins, uint64_t val,
+                 double *partials, hipStream_t s) {
+  hipLaunchKernelGGL(k_norm_masked<R>, dim3(kReduceGrid), dim3(kBlock), 0, s, psi, n_sub, ins, val,
+                     partials);
+  QS_HIP(hipGetLastError());
+}
+
+void finish_sum(const double *partials, int n_partials, int width, double *out, hipStream_t s) {
+  hipLaunchKernelGGL(k_finish_sum, dim3(1), dim3(64), 0, s, partials, n_partials, width, out);
+  QS_HIP(hipGetLastError());
+}
+
+template <typename R>
+void collapse(amp_t<R> *psi, uint64_t n, uint64_t mask, uint64_t val, double scale, hipStream_t s) {
+  hipLaunchKernelGGL(k_collapse<R>, dim3(grid_for(n)), dim3(kBlock), 0, s, psi, n, mask, val, scale);
+  QS_HIP(hipGetLastError());
+}
+
+template <typename R>
+void chunk_norms(const amp_t<R> *psi, uint64_t n, int chunk_log2, double *sums, hipStream_t s) {
+  uint64_t chunks = n >> chunk_log2;
+  hipLaunchKernelGGL(k_chunk_norms<R>, dim3((unsigned)chunks), dim3(kBlock), 0, s, psi, chunk_log2,
+                     sums);
+  QS_HIP(hipGetLastError());
+}
+
+template <typename R>
+void chunk_pick(const amp_t<R> *psi, uint64_t start, uint64_t len, double base, double rnd,
+                unsigned long long *pick, hipStream_t s) {
+  hipLaunchKernelGGL(k_chunk_pick<R>, dim3(1), dim3(kBlock), 0, s, psi, start, len, base, rnd, pick);
+  QS_HIP(hipGetLastError());
+}
+
+template <typename R>
+void classical_flags(const amp_t<R> *psi, uint64_t n, int p, double tol, int *flags, hipStream_t s) {
+  hipLaunchKernelGGL(k_classical_flags<R>, dim3(grid_for(n)), dim3(kBlock), 0, s, psi, n, p, tol,
+                     flags);
+  QS_HIP(hipGetLastError());
+}
+
+template <typename R>
+void first_pair(const amp_t<R> *psi, uint64_t n_pairs, int p, double tol, unsigned long long *key,
+                hipStream_t s) {
+  hipLaunchKernelGGL(k_first_pair<R>, dim3(grid_for(n_pairs)), dim3(kBlock), 0, s, psi, n_pairs, p,
+                     tol, key);
+  QS_HIP(hipGetLastError());
+}
+
+template <typename R>
+void drop_bit(const amp_t<R> *src, amp_t<R> *dst, uint64_t n_out, int p, int value, hipStream_t s) {
+  hipLaunchKernelGGL(k_drop_bit<R>, dim3(grid_for(n_out)), dim3(kBlock), 0, s, src, dst, n_out, p,
+                     value);
+  QS_HIP(hipGetLastError());
+}
+
+template <typename R>
+void pauli_expect(const amp_t<R> *psi, uint64_t n, uint64_t x, uint64_t z, double *partials,
+                  hipStream_t s) {
+  hipLaunchKernelGGL(k_pauli_expect<R>, dim3(kReduceGrid), dim3(kBlock), 0, s, psi, n, x, z,
+                     partials);
+  QS_HIP(hipGetLastError());
+}
+
+template <typename R>
+void pauli_sum(const amp_t<R> *src, amp_t<R> *dst, amp_t<R> *out, uint64_t n,
+               const PauliTerm *terms, int n_terms, double f_re, double f_im, bool evolve,
+               uint64_t cmask, double *partials, hipStream_t s) {
+  if (evolve)
+    hipLaunchKernelGGL((k_pauli_sum<R, true>), dim3(kReduceGrid), dim3(kBlock), 0, s, src, dst, out,
+                       n, terms, n_terms, f_re, f_im, cmask, partials);
+  else
+    hipLaunchKernelGGL((k_pauli_sum<R, false>), dim3(grid_for(n)), dim3(kBlock), 0, s, src, dst, out,
+                       n, terms, n_terms, f_re, f_im, cmask, partials);
+  QS_HIP(hipGetLastError());
+}
+
+template <typename R>
+void scale_masked(amp_t<R> *psi, uint64_t n, uint64_t cmask, double cr, double ci, hipStream_t s) {
+  hipLaunchKernelGGL(k_scale_masked<R>, dim3(grid_for(n)), dim3(kBlock), 0, s, psi, n, cmask, cr, ci);
+  QS_HIP(hipGetLastError());
+}
+
+template <typename R>
+void math_permute(const amp_t<R> *src, amp_t<R> *dst, uint64_t n, const MathArgs &args,
+                  const int32_t *table, int *err, hipStream_t s) {
+  hipLaunchKernelGGL(k_math_permute<R>, dim3(grid_for(n)), dim3(kBlock), 0, s, src, dst, n, args,
+                     table, err);
+  QS_HIP(hipGetLastError());
+}
+
+void math_injective(uint64_t n, const MathArgs &args, const int32_t *table, uint32_t *bitmap,
+                    int *collide, int *err, hipStream_t s) {
+  hipLaunchKernelGGL(k_math_injective, dim3(grid_for(n)), dim3(kBlock), 0, s, n, args, table, bitmap,
+                     collide, err);
+  QS_HIP(hipGetLastError());
+}
+
+template <typename R>
+void math_accumulate(const amp_t<R> *src, amp_t<R> *dst, uint64_t n, const MathArgs &args,
+                     const int32_t *table, int *err, hipStream_t s) {
+  if (n > (uint64_t(1) << 30))
+    throw ValueError("emulate_math: a non-injective function on more than 30 qubits is not supported");
+  uint32_t *buf = nullptr;
+  QS_HIP(hipMalloc(&buf, 4 * n * sizeof(uint32_t)));
+  uint32_t *k_in = buf, *v_in = buf + n, *k_out = buf + 2 * n, *v_out = buf + 3 * n;
+  void *tmp = nullptr;
+  size_t tmp_bytes = 0;
+  int bits = 1;
+  while ((uint64_t(1) << bits) < n) ++bits;
+  try {
+    hipLaunchKernelGGL(k_math_keys, dim3(grid_for(n)), dim3(kBlock), 0, s, n, args, table, k_in,
+                       v_in, err);
+    QS_HIP(hipGetLastError());
+    QS_HIP(hipcub::DeviceRadixSort::SortPairs(tmp, tmp_bytes, k_in, k_out, v_in, v_out, (int)n, 0,
+                                              bits, s));
+    QS_HIP(hipMalloc(&tmp, tmp_bytes));
+    QS_HIP(hipcub::DeviceRadixSort::SortPairs(tmp, tmp_bytes, k_in, k_out, v_in, v_out, (int)n, 0,
+                                              bits, s));
+    hipLaunchKernelGGL(k_math_segments<R>, dim3(grid_for(n)), dim3(kBlock), 0, s, src, dst, n, k_out,
+                       v_out);
+    QS_HIP(hipGetLastError());
+    QS_HIP(hipStreamSynchronize(s));
+  } catch (...) {
+    (void)hipStreamSynchronize(s);
+    (void)hipFree(tmp);
+    (void)hipFree(buf);
+    throw;
+  }
+  QS_HIP(hipFree(tmp));
+  QS_HIP(hipFree(buf));
+}
+
+// ------------------------------------------------------------------------------ instantiations
+#define QS_INSTANTIATE(R)                                                                          \
+  template void apply_block<R>(amp_t<R> *, uint64_t, const BitInsert &, uint64_t,                 \
+                               const uint32_t *, int, const amp_t<R> *, hipStream_t);             \
+  template void norm_masked<R>(const amp_t<R> *, uint64_t, const BitInsert &, uint64_t, double *, \
+                               hipStream_t);                                                      \
+  template void collapse<R>(amp_t<R> *, uint64_t, uint64_t, uint64_t, double, hipStream_t);      \
+  template void chunk_norms<R>(const amp_t<R> *, uint64_t, int, double *, hipStream_t);           \
+  template void chunk_pick<R>(const amp_t<R> *, uint64_t, uint64_t, double, double,               \
+                              unsigned long long *, hipStream_t);                                 \
+  template void classical_flags<R>(const amp_t<R> *, uint64_t, int, double, int *, hipStream_t);  \
+  template void first_pair<R>(const amp_t<R> *, uint64_t, int, double, unsigned long long *,      \
+                              hipStream_t);                                                       \
+  template void drop_bit<R>(const amp_t<R> *, amp_t<R> *, uint64_t, int, int, hipStream_t);       \
+  template void pauli_expect<R>(const amp_t<R> *, uint64_t, uint64_t, uint64_t, double *,         \
+                                hipStream_t);                                                     \
+  template void pauli_sum<R>(const amp_t<R> *, amp_t<R> *, amp_t<R> *, uint64_t,                  \
+                             const PauliTerm *, int, double, double, bool, uint64_t, double *,    \
+                             hipStream_t);                                                        \
+  template void scale_masked<R>(amp_t<R> *, uint64_t, uint64_t, double, double, hipStream_t);     \
+  template void math_permute<R>(const amp_t<R> *, amp_t<R> *, uint64_t, const MathArgs &,         \
+                                const int32_t *, int *, hipStream_t);                             \
+  template void math_accumulate<R>(const amp_t<R> *, amp_t<R> *, uint64_t, const MathArgs &,      \
+                                   const int32_t *, int *, hipStream_t);
+
+QS_INSTANTIATE(double)
+QS_INSTANTIATE(float)
+
+}  // namespace dev
+}  // namespace qs

@@ -1,0 +1,104 @@
+// Launchers of the HIP kernels (kernels.hip).  All take the simulator's stream; none
+// synchronises.  Index conventions: amplitude index bit p belongs to the qubit at position p.
+#pragma once
+
+#include "common.hpp"
+
+namespace qs {
+namespace dev {
+
+constexpr int kBlock = 256;       // threads per workgroup (4 wave64)
+constexpr int kReduceGrid = 2048; // workgroups of every reduction -> fixed partial-sum count
+
+// One Pauli string as  omega * X^x Z^z  with omega folded into the coefficient.
+struct PauliTerm {
+  uint64_t x, z;
+  double c_re, c_im;
+};
+
+// Register layout and constants of one emulate_math call.
+constexpr int kMaxRegs = 8;
+struct MathArgs {
+  int32_t opcode;
+  int32_t n_regs;
+  int64_t a, N;
+  uint64_t cmask;
+  uint8_t width[kMaxRegs];
+  uint8_t pos[kMaxRegs][32];
+};
+
+// dense gate block: psi[base + off(r)] = sum_c M[r][c] psi[base + off(c)] for every base with
+// target bits 0 and control bits 1.  tpos[l] is the position of matrix bit l; ins lists the
+// sorted target+control positions; n_groups = 2^(n - k - #controls).
+template <typename R>
+void apply_block(amp_t<R> *psi, uint64_t n_groups, const BitInsert &ins, uint64_t cmask,
+                 const uint32_t *tpos, int k, const amp_t<R> *d_mat, hipStream_t s);
+
+// partials[0..kReduceGrid) <- block sums of |psi_i|^2 over i = insert(g) | val, g < n_sub
+template <typename R>
+void norm_masked(const amp_t<R> *psi, uint64_t n_sub, const BitInsert &ins, uint64_t val,
+                 double *partials, hipStream_t s);
+// out[0..width) <- sum over blocks of partials[.. * width + j], fixed order (deterministic)
+void finish_sum(const double *partials, int n_partials, int width, double *out, hipStream_t s);
+
+// psi_i <- 0 where (i & mask) != val, psi_i * scale elsewhere
+template <typename R>
+void collapse(amp_t<R> *psi, uint64_t n_amps, uint64_t mask, uint64_t val, double scale,
+              hipStream_t s);
+
+// sums[c] <- sum |psi_i|^2 over chunk c of 2^chunk_log2 contiguous amplitudes
+template <typename R>
+void chunk_norms(const amp_t<R> *psi, uint64_t n_amps, int chunk_log2, double *sums,
+                 hipStream_t s);
+// *pick <- first index j in [start, start+len) with base + sum_{start<=i<=j} |psi_i|^2 >= rnd
+// (left at its initial value if none)
+template <typename R>
+void chunk_pick(const amp_t<R> *psi, uint64_t start, uint64_t len, double base, double rnd,
+                unsigned long long *pick, hipStream_t s);
+
+// flags[0] |= any |psi_i|^2 > tol with bit p clear; flags[1] likewise with bit p set
+template <typename R>
+void classical_flags(const amp_t<R> *psi, uint64_t n_amps, int p, double tol, int *flags,
+                     hipStream_t s);
+// *key <- min over pairs q (index with bit p removed) holding an entry > tol of
+//         2q + (entry with bit p clear is <= tol)
+template <typename R>
+void first_pair(const amp_t<R> *psi, uint64_t n_pairs, int p, double tol,
+                unsigned long long *key, hipStream_t s);
+// dst[j] <- src[j with a bit `value` inserted at p], j < n_out
+template <typename R>
+void drop_bit(const amp_t<R> *src, amp_t<R> *dst, uint64_t n_out, int p, int value,
+              hipStream_t s);
+
+// partials[2*b + {0,1}] <- block sums of conj(psi_i) * (-1)^popcount((i^x)&z) * psi_{i^x}
+template <typename R>
+void pauli_expect(const amp_t<R> *psi, uint64_t n_amps, uint64_t x, uint64_t z,
+                  double *partials, hipStream_t s);
+// dst_i <- f * sum_t c_t (-1)^popcount((i^x_t)&z_t) src_{i^x_t}; with `evolve` also
+// out_i += dst_i and nrm partial sums of |dst_i|^2 on entries with (i & cmask) == cmask
+template <typename R>
+void pauli_sum(const amp_t<R> *src, amp_t<R> *dst, amp_t<R> *out, uint64_t n_amps,
+               const PauliTerm *terms, int n_terms, double f_re, double f_im, bool evolve,
+               uint64_t cmask, double *partials, hipStream_t s);
+// psi_i *= c where (i & cmask) == cmask
+template <typename R>
+void scale_masked(amp_t<R> *psi, uint64_t n_amps, uint64_t cmask, double c_re, double c_im,
+                  hipStream_t s);
+
+// emulate_math: dest(i) for every i.  dst[dest(i)] = src[i] (caller guarantees injectivity);
+// err |= 1 when a result leaves int32.  With `table` the register outputs come from
+// table[packed inputs * n_regs + r] instead of the opcode.
+template <typename R>
+void math_permute(const amp_t<R> *src, amp_t<R> *dst, uint64_t n_amps, const MathArgs &args,
+                  const int32_t *table, int *err, hipStream_t s);
+// bitmap (n_amps bits, zeroed by caller) marks dest(i); collide |= 1 on a repeated target
+void math_injective(uint64_t n_amps, const MathArgs &args, const int32_t *table,
+                    uint32_t *bitmap, int *collide, int *err, hipStream_t s);
+// general (non-injective) form: dst[t] = sum over i with dest(i) = t, in increasing i
+// (stable sort of (dest, i) pairs, then ordered segment sums).  dst must be zeroed.
+template <typename R>
+void math_accumulate(const amp_t<R> *src, amp_t<R> *dst, uint64_t n_amps, const MathArgs &args,
+                     const int32_t *table, int *err, hipStream_t s);
+
+}  // namespace dev
+}  // namespace qs

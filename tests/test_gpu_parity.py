@@ -1,0 +1,193 @@
+"""Parity of the HIP library against the CPU oracle on seeded inputs (all `gpu`).
+
+complex128 must match the oracle to 1e-12 max-abs amplitude with identical seeded measurement
+outcomes; emulate_math is compared bit for bit; complex64 is held to infidelity
+1 - |<psi_ref|psi>|^2 <= 1e-6.
+"""
+
+import math
+
+import numpy as np
+import pytest
+
+from oracle.cpu_sim import OracleSimulator
+from projectq_amd import Simulator, circuits, commands
+from projectq_amd import gates as G
+
+pytestmark = pytest.mark.gpu
+
+AMP_TOL = 1e-12
+
+
+def infidelity(a, b):
+    return 1.0 - abs(np.vdot(a, b)) ** 2 / (np.vdot(a, a).real * np.vdot(b, b).real)
+
+
+def run_both(cmds, seed, fusion_flag, **kw):
+    o = OracleSimulator(seed)
+    h = Simulator(seed, **kw)
+    commands.execute(o, cmds, fusion_flag)
+    commands.execute(h, cmds, fusion_flag)
+    return o, h
+
+
+@pytest.mark.parametrize("n,layers,cnots,seed", [(6, 4, 3, 11), (10, 8, 5, 12), (14, 12, 7, 13), (20, 20, 10, 14)])
+@pytest.mark.parametrize("gate_fusion", [False, True])
+def test_random_circuit_c128(n, layers, cnots, seed, gate_fusion):
+    print(f"random circuit n={n} seed={seed} fusion={gate_fusion}", flush=True)
+    cmds = circuits.random_layers(n, layers, cnots, seed, measure=False)
+    o, h = run_both(cmds, seed, gate_fusion)
+    mo, vo = o.cheat_numpy()
+    mh, vh = h.cheat_numpy()
+    assert mo == mh
+    assert np.max(np.abs(vo - vh)) <= AMP_TOL
+    meas = circuits.measure_all(n)
+    assert commands.execute(o, meas) == commands.execute(h, meas)
+    _, vo = o.cheat_numpy()
+    _, vh = h.cheat_numpy()
+    assert np.max(np.abs(vo - vh)) <= AMP_TOL
+    for q in range(n):  # teardown as MainEngine does (deallocate every qubit)
+        o.deallocate_qubit(q)
+        h.deallocate_qubit(q)
+    assert h.cheat()[0] == {} and len(h.cheat()[1]) == 1
+
+
+@pytest.mark.parametrize("n,seed", [(12, 21), (18, 22)])
+def test_random_circuit_c64_infidelity(n, seed):
+    print(f"c64 n={n} seed={seed}", flush=True)
+    cmds = circuits.random_layers(n, 20, n // 2, seed, measure=False)
+    o, h = run_both(cmds, seed, True, precision="c64")
+    _, vo = o.cheat_numpy()
+    _, vh = h.cheat_numpy()
+    assert infidelity(vo, vh) <= 1e-6
+
+
+@pytest.mark.parametrize("k", [1, 2, 3, 4, 5])
+def test_dense_controlled_blocks(k):
+    """Random dense unitaries on k targets with 0-3 random controls, unfused (one pass each)."""
+    rng = np.random.default_rng(100 + k)
+    n = 11
+    cmds = circuits.allocate(n)
+    for _ in range(12):
+        picks = rng.choice(n, size=k + int(rng.integers(0, 4)), replace=False)
+        a = rng.normal(size=(2**k, 2**k)) + 1j * rng.normal(size=(2**k, 2**k))
+        u, _ = np.linalg.qr(a)
+        cmds.append(commands.gate(u, [int(q) for q in picks[:k]], [int(q) for q in picks[k:]]))
+        cmds.append(commands.gate(G.H, [int(rng.integers(0, n))]))
+    for gate_fusion in (False, True):
+        o, h = run_both(cmds, 5, gate_fusion)
+        assert np.max(np.abs(o.cheat_numpy()[1] - h.cheat_numpy()[1])) <= AMP_TOL
+
+
+@pytest.mark.parametrize("fusion", ["window"])
+def test_window_fusion_matches_unfused_oracle(fusion):
+    cmds = circuits.mixed_circuit(14, 10, 31)
+    o = OracleSimulator(3)
+    commands.execute(o, cmds, False)
+    h = Simulator(3, fusion=fusion)
+    commands.execute(h, cmds, True)
+    assert np.max(np.abs(o.cheat_numpy()[1] - h.cheat_numpy()[1])) <= AMP_TOL
+
+
+def _superposition(sim, n, seed):
+    """Allocate n qubits and load one seeded random normalised state (bit-identical inputs)."""
+    rng = np.random.default_rng(seed)
+    for q in range(n):
+        sim.allocate_qubit(q)
+    wf = rng.normal(size=2**n) + 1j * rng.normal(size=2**n)
+    wf /= np.linalg.norm(wf)
+    sim.set_wavefunction(wf, list(range(n)))
+
+
+MATH_CASES = [
+    ("add_quantum", lambda: G.add_quantum(4), [[0, 1, 2, 3], [4, 5, 6, 7]], []),
+    ("add_quantum_carry", lambda: G.add_quantum(3), [[0, 1, 2], [3, 4, 5], [6]], [7]),
+    ("subtract", G.subtract_quantum, [[0, 1, 2], [3, 4, 5]], [6]),
+    ("comparator", G.comparator_quantum, [[0, 1, 2], [3, 4, 5], [6]], []),
+    ("multiply", G.multiply_quantum, [[0, 1], [2, 3], [4, 5, 6, 7, 8]], []),
+    ("divide", G.divide_quantum, [[0, 1, 2], [3, 4, 5], [6, 7, 8]], []),
+    ("python_lambda", lambda: (lambda x: [(3 * x[0] + 1) % 8, x[1]]), [[0, 1, 2], [5, 4]], [8]),
+    ("non_injective", lambda: (lambda x: [x[0] // 2, x[1]]), [[0, 1, 2], [3, 4]], []),
+    ("add_const_mod_collisions", lambda: (lambda x: [(x[0] + 3) % 5]), [[1, 3, 5]], [0]),
+]
+
+
+@pytest.mark.parametrize("name,fn,regs,ctrl", MATH_CASES, ids=[c[0] for c in MATH_CASES])
+def test_emulate_math_bit_exact(name, fn, regs, ctrl):
+    n = 9
+    o, h = OracleSimulator(1), Simulator(1)
+    _superposition(o, n, 5)
+    _superposition(h, n, 5)
+    o.emulate_math(fn(), regs, ctrl)
+    h.emulate_math(fn(), regs, ctrl)
+    vo, vh = o.cheat_numpy()[1], h.cheat_numpy()[1]
+    # identical inputs and a pure permutation of amplitudes: bit for bit
+    assert np.array_equal(vo, vh)
+
+
+@pytest.mark.parametrize("which,a,N", [("add", 5, 0), ("add", -3, 0), ("add_mod", 4, 13), ("mul_mod", 7, 15)])
+def test_emulate_math_constants(which, a, N):
+    n = 8
+    o, h = OracleSimulator(1), Simulator(1)
+    for sim in (o, h):
+        _superposition(sim, n, 77)
+        commands.execute(sim, [("math_const", which, a, N, [[0, 1, 2, 3]], [7])])
+    assert np.array_equal(o.cheat_numpy()[1], h.cheat_numpy()[1])
+
+
+def test_expectation_and_operator():
+    n = 12
+    o, h = OracleSimulator(1), Simulator(1)
+    _superposition(o, n, 9)
+    _superposition(h, n, 9)
+    terms = circuits.random_pauli_operator(n, 30, 4)
+    ids = list(range(n))
+    eo, eh = o.get_expectation_value(terms, ids), h.get_expectation_value(terms, ids)
+    assert abs(eo - eh) <= 1e-12 * max(1.0, abs(eo))
+    cterms = [(t, c * (1 + 0.5j)) for t, c in terms[:10]]
+    o.apply_qubit_operator(cterms, ids)
+    h.apply_qubit_operator(cterms, ids)
+    assert np.max(np.abs(o.cheat_numpy()[1] - h.cheat_numpy()[1])) <= 1e-11
+
+
+def test_time_evolution_controlled():
+    n = 9
+    o, h = OracleSimulator(1), Simulator(1)
+    _superposition(o, n, 3)
+    _superposition(h, n, 3)
+    terms = circuits.random_pauli_operator(6, 5, 8) + [([], 0.7)]
+    for sim in (o, h):
+        sim.emulate_time_evolution(terms, 0.6, list(range(6)), [7])
+    assert np.max(np.abs(o.cheat_numpy()[1] - h.cheat_numpy()[1])) <= 1e-11
+
+
+def test_probability_amplitude_collapse():
+    n = 13
+    o, h = OracleSimulator(1), Simulator(1)
+    _superposition(o, n, 17)
+    _superposition(h, n, 17)
+    rng = np.random.default_rng(2)
+    for _ in range(8):
+        ids = [int(q) for q in rng.choice(n, size=5, replace=False)]
+        bits = [bool(b) for b in rng.integers(0, 2, size=5)]
+        assert abs(o.get_probability(bits, ids) - h.get_probability(bits, ids)) <= 1e-13
+    bits = [bool(b) for b in rng.integers(0, 2, size=n)]
+    assert abs(o.get_amplitude(bits, list(range(n))) - h.get_amplitude(bits, list(range(n)))) <= AMP_TOL
+    o.collapse_wavefunction([1, 4, 7], [1, 0, 1])
+    h.collapse_wavefunction([1, 4, 7], [1, 0, 1])
+    assert np.max(np.abs(o.cheat_numpy()[1] - h.cheat_numpy()[1])) <= AMP_TOL
+    for q in (0, 5):
+        assert o.is_classical(q, 1e-12) == h.is_classical(q, 1e-12)
+
+
+def test_measurement_sequence_seeded():
+    """Seeded outcomes over many measurements equal the oracle's (same mt19937 stream)."""
+    for seed in (1, 2, 3, 42, 4242):
+        n = 12
+        o, h = OracleSimulator(seed), Simulator(seed)
+        _superposition(o, n, seed)
+        _superposition(h, n, seed)
+        order = list(np.random.default_rng(seed).permutation(n))
+        ro = [o.measure_qubits([int(q)])[0] for q in order]
+        rh = [h.measure_qubits([int(q)])[0] for q in order]
+        assert ro == rh
