@@ -1,0 +1,240 @@
+#!/usr/bin/env python3
+"""Throughput of the state-vector simulator on one MI355X; prints ONE JSON line.
+
+Workload: the 32-qubit depth-100 random circuit (per layer: H on every qubit, one random
+Rx/Ry/Rz per qubit with angle U[0, 2pi), 16 CNOTs on random distinct pairs), complex64 state,
+gates queued in a window and partitioned by the planner.  A step is one circuit layer
+(80 gates); the timed region is K layers with the state already resident in HBM.
+
+Also reported:
+* plain complex128 path (the reference's default: one pass per gate, run() after every gate)
+  timed on a few layers of the same circuit, and the speed-up of the headline over it;
+* roofline of the dominant kernel (the gate pass): algorithmic bytes / summed HIP-event time;
+* a CPU baseline: ProjectQ's own C++ simulator built from its sources (oracle/_ref), or the
+  in-repo oracle where that build is absent, on a bounded sample of the same circuit.
+"""
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "gates/s + % of 8 TB/s HBM at 32q; c64+learned fusion vs plain c128 on GPU"
+HBM_PEAK_GBS = 8000.0
+CNOTS_PER_LAYER = 16
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def layer_slices(cmds, n):
+    """Split random_layers() output into (allocation commands, per-layer gate lists)."""
+    alloc = cmds[:n]
+    gates = cmds[n:]
+    per = 2 * n + CNOTS_PER_LAYER
+    return alloc, [gates[i : i + per] for i in range(0, len(gates), per)]
+
+
+def submit(sim, layer, run_each):
+    for _, m, targets, controls in layer:
+        sim.apply_controlled_gate(m, list(targets), list(controls))
+        if run_each:
+            sim.run()
+
+
+def gpu_headline(args, cmds, n):
+    from projectq_amd import Simulator
+
+    _, layers = layer_slices(cmds, n)
+    sim = Simulator(args.seed, precision=args.precision, fusion="window", planner=args.planner, capacity_qubits=n)
+    for q in range(n):
+        sim.allocate_qubit(q)
+    for layer in layers[: args.warmup]:
+        submit(sim, layer, run_each=False)
+    sim.run()
+    sim.synchronize()
+    log(f"headline: warm-up {args.warmup} layers done")
+    sim.reset_stats()
+    sim.set_profiling(True)
+    timed = layers[args.warmup : args.warmup + args.steps]
+    t0 = time.perf_counter()
+    for layer in timed:
+        submit(sim, layer, run_each=False)
+    sim.run()
+    sim.synchronize()
+    t1 = time.perf_counter()
+    stats = sim.stats()
+    n_gates = sum(len(layer) for layer in timed)
+    log(f"headline: {len(timed)} layers, {n_gates} gates in {t1 - t0:.3f}s, {stats['passes']} passes")
+    del sim
+    return t1 - t0, n_gates, stats
+
+
+def gpu_plain_c128(args, cmds, n):
+    from projectq_amd import Simulator
+
+    _, layers = layer_slices(cmds, n)
+    sim = Simulator(args.seed, precision="c128", fusion="reference", capacity_qubits=n)
+    for q in range(n):
+        sim.allocate_qubit(q)
+    submit(sim, layers[0][: 2 * n], run_each=True)  # warm-up: one layer's single-qubit gates
+    sim.synchronize()
+    sim.reset_stats()
+    sim.set_profiling(True)
+    k = args.plain_layers
+    t0 = time.perf_counter()
+    for layer in layers[1 : 1 + k]:
+        submit(sim, layer, run_each=True)
+    sim.synchronize()
+    t1 = time.perf_counter()
+    stats = sim.stats()
+    log(f"plain c128: {k} layers in {t1 - t0:.3f}s ({stats['passes']} passes)")
+    del sim
+    return (t1 - t0) / k, stats
+
+
+def cpu_baseline(args):
+    """ProjectQ's C++ simulator (oracle/_ref) on a bounded sample: `cpu_layers` layers of the same
+    circuit family at `cpu_qubits` qubits, gate_fusion off (the reference default).  Per-gate cost
+    scales with 2^n, so gates/s is also given scaled to the headline's qubit count."""
+    from oracle import ref
+
+    from projectq_amd import circuits, commands
+
+    mod = ref.load("intrin")
+    kind = "reference"
+    if mod is None:
+        from oracle.cpu_sim import OracleSimulator as make
+
+        kind = "port"
+    else:
+        make = mod.Simulator
+    nq = args.cpu_qubits
+    cmds = circuits.random_layers(nq, args.cpu_layers, CNOTS_PER_LAYER, args.seed, measure=False)
+    sim = make(args.seed)
+    commands.execute(sim, cmds[:nq])
+    gates = cmds[nq:]
+    t0 = time.perf_counter()
+    commands.execute(sim, gates, gate_fusion=False)
+    t1 = time.perf_counter()
+    rate = len(gates) / (t1 - t0)
+    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    log(f"cpu baseline ({kind}): {len(gates)} gates at {nq} qubits in {t1 - t0:.2f}s")
+    return {
+        "value": rate * 2.0 ** (nq - args.qubits),
+        "unit": "gates/s",
+        "cores": cores,
+        "kind": kind,
+        "sample": (
+            f"{args.cpu_layers} layer(s) ({len(gates)} gates) of the same random-circuit family at {nq} qubits, "
+            f"complex128, gate_fusion off, OpenMP {cores} threads: {rate:.1f} gates/s measured, "
+            f"scaled by 2^({nq}-{args.qubits}) to {args.qubits} qubits"
+        ),
+        "measured_gates_per_s": rate,
+        "measured_qubits": nq,
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=100, help="timed circuit layers")
+    ap.add_argument("--warmup", type=int, default=2, help="untimed circuit layers before the timed ones")
+    ap.add_argument("--qubits", type=int, default=32)
+    ap.add_argument("--precision", default="c64", choices=["c64", "c128"])
+    ap.add_argument("--planner", default="greedy", choices=["greedy", "learned"])
+    ap.add_argument("--plain-layers", type=int, default=2)
+    ap.add_argument("--cpu-qubits", type=int, default=28)
+    ap.add_argument("--cpu-layers", type=int, default=1)
+    ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--seed", type=int, default=2024)
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(
+            "bench.py measures one GPU: this build has no multi-GPU path (the state lives in one GPU's HBM); "
+            "run with --gpus 1",
+            file=sys.stderr,
+        )
+        sys.exit(2)
+
+    from projectq_amd import circuits
+
+    n = args.qubits
+    total_layers = args.warmup + args.steps
+    cmds = circuits.random_layers(n, max(total_layers, args.plain_layers + 1), CNOTS_PER_LAYER, args.seed, measure=False)
+    log(f"circuit: {n} qubits, {total_layers} layers, {CNOTS_PER_LAYER} CNOTs/layer")
+
+    wall, n_gates, stats = gpu_headline(args, cmds, n)
+    value = n_gates / wall
+    ms_per_step = 1e3 * wall / args.steps
+
+    achieved = None
+    if stats["timed_passes"] and stats["gpu_ms"] > 0:
+        achieved = stats["bytes"] / (stats["gpu_ms"] * 1e-3) / 1e9
+    roofline = {
+        "bound": "hbm",
+        "achieved": round(achieved, 1) if achieved else None,
+        "peak": HBM_PEAK_GBS,
+        "unit": "GB/s",
+        "frac": round(achieved / HBM_PEAK_GBS, 4) if achieved else None,
+        "traffic": None,
+        "kernel": "k_apply (dense fused-block gate pass)",
+        "passes": stats["passes"],
+        "bytes_per_pass": stats["bytes"] / max(stats["passes"], 1),
+        "avg_pass_ms": stats["gpu_ms"] / max(stats["timed_passes"], 1),
+    }
+
+    out = {
+        "metric": METRIC,
+        "value": round(value, 2),
+        "unit": "gates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32" if args.precision == "c64" else "fp64",
+        "data": "synthetic (seeded random circuit)",
+        "config": {
+            "workload": (
+                f"random circuit, {n} qubits, depth {args.steps} timed (+{args.warmup} warm-up): per layer H and one "
+                f"random Rx/Ry/Rz per qubit plus {CNOTS_PER_LAYER} CNOTs; {args.precision} state, window fusion, "
+                f"{args.planner} planner"
+            ),
+            "qubits": n,
+            "layers": args.steps,
+            "gates_per_layer": 2 * n + CNOTS_PER_LAYER,
+            "precision": args.precision,
+            "planner": args.planner,
+            "parallelism": "single GPU",
+        },
+        "roofline": roofline,
+    }
+
+    if not args.no_plain:
+        plain_ms_layer, pstats = gpu_plain_c128(args, cmds, n)
+        out["plain_c128"] = {
+            "ms_per_layer": round(1e3 * plain_ms_layer, 3),
+            "gates_per_s": round((2 * n + CNOTS_PER_LAYER) / plain_ms_layer, 2),
+            "layers_timed": args.plain_layers,
+            "achieved_gbs": round(pstats["bytes"] / (pstats["gpu_ms"] * 1e-3) / 1e9, 1) if pstats["gpu_ms"] else None,
+        }
+        out["speedup_vs_plain_c128"] = round((1e3 * plain_ms_layer) / ms_per_step, 3)
+
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(args)
+
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

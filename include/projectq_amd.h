@@ -142,6 +142,24 @@ int qs_run(qs_simulator *sim);
 int qs_cheat_sizes(qs_simulator *sim, size_t *n_qubits, size_t *n_amps);
 int qs_cheat(qs_simulator *sim, uint32_t *ids, uint32_t *positions, double *state);
 
+/* ---- extensions (no counterpart in the reference) ------------------------------------------
+ * Used by measurement harnesses and by tests that compare precisions on one GPU. */
+
+/* Block until every queued device operation of this simulator has finished. */
+int qs_synchronize(qs_simulator *sim);
+/* Record HIP events around every gate pass (enabled != 0) so qs_get_stats can report the GPU
+ * time of the passes; off by default. */
+int qs_set_profiling(qs_simulator *sim, int enabled);
+/* Counters since creation or the last qs_reset_stats: gate passes launched, their algorithmic
+ * HBM bytes (2 * bytes per amplitude * amplitudes touched), and -- for passes launched with
+ * profiling on -- the summed GPU time in ms and how many passes it covers.  Synchronises. */
+int qs_get_stats(qs_simulator *sim, uint64_t *passes, double *bytes, double *gpu_ms,
+                 uint64_t *timed_passes);
+int qs_reset_stats(qs_simulator *sim);
+/* <a|b> over two simulators on the same GPU holding the same qubits at the same positions
+ * (precisions may differ); accumulated in FP64.  result = (re, im). */
+int qs_overlap(qs_simulator *a, qs_simulator *b, double *result);
+
 #ifdef __cplusplus
 }
 #endif

@@ -807,4 +807,26 @@ API int qs_cheat(qs_simulator *s, uint32_t *ids, uint32_t *positions, double *st
   });
 }
 
+API int qs_synchronize(qs_simulator *) { return QS_OK; }
+API int qs_set_profiling(qs_simulator *, int) { return QS_OK; }
+API int qs_get_stats(qs_simulator *, uint64_t *passes, double *bytes, double *ms, uint64_t *timed) {
+  *passes = 0;
+  *bytes = 0.0;
+  *ms = 0.0;
+  *timed = 0;
+  return QS_OK;
+}
+API int qs_reset_stats(qs_simulator *) { return QS_OK; }
+API int qs_overlap(qs_simulator *a, qs_simulator *b, double *out) {
+  return guarded([&] {
+    a->o.run();
+    b->o.run();
+    if (a->o.pos != b->o.pos) throw std::invalid_argument("overlap: different qubits or positions");
+    cplx acc = 0.0;
+    for (size_t i = 0; i < a->o.psi.size(); ++i) acc += std::conj(a->o.psi[i]) * b->o.psi[i];
+    out[0] = acc.real();
+    out[1] = acc.imag();
+  });
+}
+
 }  // extern "C"

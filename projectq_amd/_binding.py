@@ -95,6 +95,14 @@ _SIGNATURES = {
     "qs_run": (ctypes.c_int, [ctypes.c_void_p]),
     "qs_cheat_sizes": (ctypes.c_int, [ctypes.c_void_p, _szp, _szp]),
     "qs_cheat": (ctypes.c_int, [ctypes.c_void_p, _u32p, _u32p, _f64p]),
+    "qs_synchronize": (ctypes.c_int, [ctypes.c_void_p]),
+    "qs_set_profiling": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
+    "qs_get_stats": (
+        ctypes.c_int,
+        [ctypes.c_void_p, ctypes.POINTER(ctypes.c_uint64), _f64p, _f64p, ctypes.POINTER(ctypes.c_uint64)],
+    ),
+    "qs_reset_stats": (ctypes.c_int, [ctypes.c_void_p]),
+    "qs_overlap": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, _f64p]),
 }
 
 #: every symbol the C ABI declares (include/projectq_amd.h)
@@ -422,6 +430,34 @@ class CAbiSimulator:
 
     def run(self):
         self._check(self._lib.qs_run(self._handle))
+
+    # ------------------------------------------------------------------ extensions
+    def synchronize(self):
+        """Wait for every queued device operation (gate passes run asynchronously)."""
+        self._check(self._lib.qs_synchronize(self._handle))
+
+    def set_profiling(self, enabled=True):
+        self._check(self._lib.qs_set_profiling(self._handle, 1 if enabled else 0))
+
+    def stats(self):
+        """dict(passes, bytes, gpu_ms, timed_passes) of the gate passes since the last reset."""
+        passes, timed = ctypes.c_uint64(), ctypes.c_uint64()
+        nbytes, ms = ctypes.c_double(), ctypes.c_double()
+        self._check(
+            self._lib.qs_get_stats(self._handle, ctypes.byref(passes), ctypes.byref(nbytes), ctypes.byref(ms), ctypes.byref(timed))
+        )
+        return {"passes": passes.value, "bytes": nbytes.value, "gpu_ms": ms.value, "timed_passes": timed.value}
+
+    def reset_stats(self):
+        self._check(self._lib.qs_reset_stats(self._handle))
+
+    def overlap(self, other):
+        """<self|other> computed on the GPU (same qubits at the same positions)."""
+        if other._lib is not self._lib:
+            raise ValueError("overlap needs two simulators of the same library")
+        out = np.zeros(2, dtype=np.float64)
+        self._check(self._lib.qs_overlap(self._handle, other._handle, _ptr(out, ctypes.c_double)))
+        return complex(out[0], out[1])
 
     def cheat_numpy(self):
         """cheat() without the Python-list conversion: (dict id->position, complex128 ndarray)."""

@@ -204,3 +204,24 @@ API int qs_cheat_sizes(qs_simulator *s, size_t *n_qubits, size_t *n_amps) {
 API int qs_cheat(qs_simulator *s, uint32_t *ids, uint32_t *positions, double *state) {
   return guarded(s, [&] { s->engine.cheat(ids, positions, state); });
 }
+
+API int qs_synchronize(qs_simulator *s) {
+  return guarded(s, [&] { s->engine.synchronize(); });
+}
+API int qs_set_profiling(qs_simulator *s, int enabled) {
+  return guarded(s, [&] { s->engine.set_profiling(enabled != 0); });
+}
+API int qs_get_stats(qs_simulator *s, uint64_t *passes, double *bytes, double *gpu_ms, uint64_t *timed) {
+  return guarded(s, [&] { s->engine.get_stats(passes, bytes, gpu_ms, timed); });
+}
+API int qs_reset_stats(qs_simulator *s) {
+  return guarded(s, [&] { s->engine.reset_stats(); });
+}
+API int qs_overlap(qs_simulator *a, qs_simulator *b, double *result) {
+  return guarded(a, [&] {
+    if (a->device != b->device) throw qs::ValueError("overlap: the simulators live on different GPUs");
+    qs::cplx v = a->engine.overlap(b->engine);
+    result[0] = v.real();
+    result[1] = v.imag();
+  });
+}

@@ -62,6 +62,11 @@ Engine::Engine(uint32_t seed, int precision, int fusion_mode, int planner, unsig
 
 Engine::~Engine() {
   if (stream_) (void)hipStreamSynchronize(stream_);
+  for (auto &p : ev_used_) {
+    (void)hipEventDestroy(p.first);
+    (void)hipEventDestroy(p.second);
+  }
+  for (auto e : ev_free_) (void)hipEventDestroy(e);
   (void)hipFree(psi_);
   release_scratch();
   (void)hipFree(d_partials_);
@@ -377,8 +382,72 @@ void Engine::exec_block(const Block &b) {
       m[i].y = (R)b.m[i].imag();
     }
     const amp_t<R> *dm = (const amp_t<R> *)upload(m.data(), m.size() * sizeof(amp_t<R>));
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (profiling_) {
+      e0 = take_event();
+      e1 = take_event();
+      QS_HIP(hipEventRecord(e0, stream_));
+    }
     dev::apply_block<R>((amp_t<R> *)psi_, n_groups, ins, cmask, tpos.data(), k, dm, stream_);
+    if (profiling_) {
+      QS_HIP(hipEventRecord(e1, stream_));
+      ev_used_.push_back({e0, e1});
+    }
   });
+  // algorithmic traffic: every touched amplitude is read once and written once
+  count_pass(2.0 * (double)amp_bytes() * (double)(n_groups << k));
+}
+
+// ------------------------------------------------------------------------------ measurement hooks
+void Engine::synchronize() { sync(); }
+
+hipEvent_t Engine::take_event() {
+  if (!ev_free_.empty()) {
+    hipEvent_t e = ev_free_.back();
+    ev_free_.pop_back();
+    return e;
+  }
+  hipEvent_t e;
+  QS_HIP(hipEventCreate(&e));
+  return e;
+}
+
+void Engine::harvest_events() {
+  sync();
+  for (auto &p : ev_used_) {
+    float ms = 0.f;
+    QS_HIP(hipEventElapsedTime(&ms, p.first, p.second));
+    stat_ms_ += ms;
+    stat_timed_++;
+    ev_free_.push_back(p.first);
+    ev_free_.push_back(p.second);
+  }
+  ev_used_.clear();
+}
+
+void Engine::get_stats(uint64_t *passes, double *bytes, double *gpu_ms, uint64_t *timed) {
+  harvest_events();
+  *passes = stat_passes_;
+  *bytes = stat_bytes_;
+  *gpu_ms = stat_ms_;
+  *timed = stat_timed_;
+}
+
+void Engine::reset_stats() {
+  harvest_events();
+  stat_passes_ = stat_timed_ = 0;
+  stat_bytes_ = stat_ms_ = 0.0;
+}
+
+cplx Engine::overlap(Engine &other) {
+  run();
+  other.run();
+  other.sync();
+  if (other.pos_ != pos_) throw ValueError("overlap: the simulators hold different qubits or positions");
+  dev::overlap(psi_, c64_, other.psi_, other.c64_, amps(), d_partials_, stream_);
+  double im;
+  double re = finish(dev::kReduceGrid, 2, &im);
+  return cplx(re, im);
 }
 
 // ------------------------------------------------------------------------------ emulate_math

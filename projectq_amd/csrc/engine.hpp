@@ -59,6 +59,13 @@ class Engine {
 
   // execute one fused block (also used by the planners)
   void exec_block(const Block &b);
+
+  // measurement hooks (C-ABI extensions)
+  void synchronize();
+  void set_profiling(bool on) { profiling_ = on; }
+  void get_stats(uint64_t *passes, double *bytes, double *gpu_ms, uint64_t *timed);
+  void reset_stats();
+  cplx overlap(Engine &other);
   unsigned num_qubits() const { return n_; }
   unsigned position(unsigned id) const { return pos_.at(id); }
   hipStream_t stream() const { return stream_; }
@@ -107,6 +114,17 @@ class Engine {
   unsigned long long *d_key_ = nullptr, *h_key_ = nullptr;
   double *d_chunks_ = nullptr, *h_chunks_ = nullptr;
   size_t chunks_cap_ = 0;
+  bool profiling_ = false;
+  uint64_t stat_passes_ = 0, stat_timed_ = 0;
+  double stat_bytes_ = 0.0, stat_ms_ = 0.0;
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> ev_used_;
+  std::vector<hipEvent_t> ev_free_;
+  hipEvent_t take_event();
+  void harvest_events();
+  void count_pass(double bytes) {
+    stat_passes_++;
+    stat_bytes_ += bytes;
+  }
   char *d_arena_ = nullptr, *h_arena_ = nullptr;
   size_t arena_off_ = 0;
   static constexpr size_t kArenaBytes = size_t(8) << 20;

@@ -526,9 +526,43 @@ __global__ __launch_bounds__(kBlock) void k_math_segments(const amp_t<R> *__rest
   }
 }
 
+template <typename Ra, typename Rb>
+__global__ __launch_bounds__(kBlock) void k_overlap(const amp_t<Ra> *__restrict__ a,
+                                                    const amp_t<Rb> *__restrict__ b, uint64_t n,
+                                                    double *partials) {
+  __shared__ double lds[kBlock / 64];
+  Kahan sr, si;
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+    amp_t<Ra> x = a[i];
+    amp_t<Rb> y = b[i];
+    sr.add(re(x) * re(y) + im(x) * im(y));
+    si.add(re(x) * im(y) - im(x) * re(y));
+  }
+  double tr = block_sum(sr.value(), lds);
+  double ti = block_sum(si.value(), lds);
+  if (threadIdx.x == 0) {
+    partials[2 * blockIdx.x] = tr;
+    partials[2 * blockIdx.x + 1] = ti;
+  }
+}
+
 }  // namespace
 
 // ------------------------------------------------------------------------------ launchers
+void overlap(const void *a, bool a_c64, const void *b, bool b_c64, uint64_t n, double *partials,
+             hipStream_t s) {
+  dim3 g(kReduceGrid), t(kBlock);
+  if (!a_c64 && !b_c64)
+    hipLaunchKernelGGL((k_overlap<double, double>), g, t, 0, s, (const double2 *)a, (const double2 *)b, n, partials);
+  else if (!a_c64 && b_c64)
+    hipLaunchKernelGGL((k_overlap<double, float>), g, t, 0, s, (const double2 *)a, (const float2 *)b, n, partials);
+  else if (a_c64 && !b_c64)
+    hipLaunchKernelGGL((k_overlap<float, double>), g, t, 0, s, (const float2 *)a, (const double2 *)b, n, partials);
+  else
+    hipLaunchKernelGGL((k_overlap<float, float>), g, t, 0, s, (const float2 *)a, (const float2 *)b, n, partials);
+  QS_HIP(hipGetLastError());
+}
 template <typename R>
 void apply_block(amp_t<R> *psi, uint64_t n_groups, const BitInsert &ins, uint64_t cmask,
                  const uint32_t *tpos, int k, const amp_t<R> *d_mat, hipStream_t s) {

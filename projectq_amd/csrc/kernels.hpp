@@ -100,5 +100,9 @@ template <typename R>
 void math_accumulate(const amp_t<R> *src, amp_t<R> *dst, uint64_t n_amps, const MathArgs &args,
                      const int32_t *table, int *err, hipStream_t s);
 
+// partials[2*b + {0,1}] <- block sums of conj(a_i) * b_i (either operand c128 or c64)
+void overlap(const void *a, bool a_c64, const void *b, bool b_c64, uint64_t n_amps,
+             double *partials, hipStream_t s);
+
 }  // namespace dev
 }  // namespace qs
