@@ -147,7 +147,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=2, help="untimed circuit layers before the timed ones")
     ap.add_argument("--qubits", type=int, default=32)
     ap.add_argument("--precision", default="c64", choices=["c64", "c128"])
-    ap.add_argument("--planner", default="greedy", choices=["greedy", "learned"])
+    ap.add_argument("--planner", default="tile", choices=["greedy", "tile", "learned"])
     ap.add_argument("--plain-layers", type=int, default=2)
     ap.add_argument("--cpu-qubits", type=int, default=28)
     ap.add_argument("--cpu-layers", type=int, default=1)
@@ -185,7 +185,7 @@ def main():
         "unit": "GB/s",
         "frac": round(achieved / HBM_PEAK_GBS, 4) if achieved else None,
         "traffic": None,
-        "kernel": "k_apply (dense fused-block gate pass)",
+        "kernel": "k_tile_pass (LDS tile pass)" if args.planner != "greedy" else "k_apply (dense fused-block gate pass)",
         "passes": stats["passes"],
         "bytes_per_pass": stats["bytes"] / max(stats["passes"], 1),
         "avg_pass_ms": stats["gpu_ms"] / max(stats["timed_passes"], 1),

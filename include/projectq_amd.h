@@ -56,7 +56,10 @@ typedef struct qs_simulator qs_simulator;
 
 /* planner used for QS_FUSION_WINDOW */
 #define QS_PLANNER_GREEDY 0   /* greedy max-5 blocks (fusion.hpp restated over the whole window) */
-#define QS_PLANNER_LEARNED 1  /* MLP cost model chooses block boundaries; greedy is its fallback */
+#define QS_PLANNER_LEARNED 1  /* tile passes whose tiles an MLP cost model picks; QS_PLANNER_TILE's
+                                 heuristic is its fallback */
+#define QS_PLANNER_TILE 2     /* tile passes: every gate whose targets lie in a 2^t-amplitude LDS
+                                 tile is applied during one streaming pass (DAG order kept) */
 
 /* emulate_math opcodes (qs_emulate_math_op).  The first three are the reference's native fast
  * paths (simulator.hpp:271-290); the rest are the integer functions of projectq/libs/math/_gates.py,

@@ -25,7 +25,7 @@ QS_ERR_CALLBACK = 5
 
 PRECISIONS = {"c128": 0, "c64": 1}
 FUSION_MODES = {"reference": 0, "window": 1}
-PLANNERS = {"greedy": 0, "learned": 1}
+PLANNERS = {"greedy": 0, "learned": 1, "tile": 2}
 
 _u32p = ctypes.POINTER(ctypes.c_uint32)
 _u8p = ctypes.POINTER(ctypes.c_uint8)

@@ -6,7 +6,9 @@
 * precision="c128" (the reference's complex<double>) or "c64" (complex<float> state, FP64 sums);
 * fusion="reference" (the reference's greedy min-4/max-5 rule, simulator.hpp:204-222) or
   "window" (gates queue until run(); a planner partitions the whole window);
-* planner="greedy" or "learned" (window mode only);
+* planner="greedy" (the reference's block rule, one pass per block), "tile" (LDS tile passes,
+  many gates per pass) or "learned" (tile passes chosen with a learned cost model) -- window
+  mode only;
 * capacity_qubits=N reserves HBM for N qubits up front.
 
 There is no CPU fallback: without the HIP library this module raises ImportError.

@@ -8,6 +8,8 @@
 #include <stdexcept>
 #include <string>
 
+#include "layout.hpp"
+
 namespace qs {
 
 // Error classes surfaced through the C ABI (QS_ERR_* in include/projectq_amd.h).
@@ -47,15 +49,5 @@ struct Amp<float> {
 };
 template <typename R>
 using amp_t = typename Amp<R>::type;
-
-// Largest number of bit positions a kernel argument can name (targets + controls + ...).
-constexpr int kMaxBits = 48;
-
-// Sorted list of bit positions at which a zero is inserted into a compact counter; used to
-// enumerate every index whose chosen bits are fixed (targets 0, controls 1, ...).
-struct BitInsert {
-  uint32_t count;
-  uint8_t pos[kMaxBits];
-};
 
 }  // namespace qs

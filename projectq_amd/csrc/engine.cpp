@@ -36,7 +36,7 @@ Engine::Engine(uint32_t seed, int precision, int fusion_mode, int planner, unsig
       cap_hint_(capacity_hint), rng_(seed) {
   if (precision != 0 && precision != 1) throw ValueError("precision must be QS_C128 or QS_C64");
   if (fusion_mode != 0 && fusion_mode != 1) throw ValueError("unknown fusion mode");
-  if (planner != 0 && planner != 1) throw ValueError("unknown planner");
+  if (planner < 0 || planner > 2) throw ValueError("unknown planner");
   if (capacity_hint > 36) throw ValueError("capacity hint above 36 qubits");
   QS_HIP(hipGetDevice(&device_));
   QS_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
@@ -337,8 +337,14 @@ void Engine::run() {
 // simulator.hpp:204-222) applied gate by gate; each block is one streaming pass.
 void Engine::flush_window() {
   if (window_.empty()) return;
+  for (const Gate &g : window_)
+    if (g.ids.empty() || g.ids.size() > 5) throw ValueError("Gates with more than 5 qubits are not supported!");
   std::vector<Gate> gates;
   gates.swap(window_);
+  if (planner_kind_ != 0) {
+    flush_window_tiles(gates);
+    return;
+  }
   GateQueue q;
   for (const Gate &g : gates) {
     GateQueue trial = q;
@@ -356,6 +362,62 @@ void Engine::flush_window() {
     }
   }
   if (!q.empty()) exec_block(q.fuse());
+}
+
+// Tile size: 2^t amplitudes of LDS per workgroup (64 KiB), at most n bits, and small enough
+// that a state of n qubits still spreads over >= 256 tiles when n is large.
+unsigned Engine::tile_bits() const {
+  const unsigned tmax = c64_ ? 13u : 12u;
+  if (n_ <= 10) return n_;
+  return std::min(tmax, std::max(10u, n_ - 8));
+}
+
+void Engine::flush_window_tiles(std::vector<Gate> &gates) {
+  std::vector<PGate> pg;
+  pg.reserve(gates.size());
+  for (const Gate &g : gates) {
+    PGate p;
+    p.m = g.m;
+    for (unsigned id : g.ids) p.tpos.push_back(pos_.at(id));
+    for (unsigned c : g.ctrl) p.cmask |= uint64_t(1) << pos_.at(c);
+    if (p.tmask() & p.cmask) continue;  // a control that is also a target: never applies
+    pg.push_back(std::move(p));
+  }
+  pg = merge_single_qubit_runs(pg);
+  TileShape shape{n_, tile_bits(), c64_ ? 5u : 4u};
+  std::vector<PPass> passes = schedule_tiles(pg, shape);
+  for (const PPass &ps : passes) exec_tile_pass(pg, ps, shape.t);
+}
+
+void Engine::exec_tile_pass(const std::vector<PGate> &gates, const PPass &ps, unsigned t) {
+  dev::TilePass P{};
+  std::vector<dev::TileOp> ops;
+  std::vector<cplx> mats;
+  build_tile_pass(gates, ps, t, P, ops, mats);
+  if (ops.empty()) return;
+  const uint64_t n_tiles = amps() >> t;
+  const dev::TileOp *d_ops = (const dev::TileOp *)upload(ops.data(), ops.size() * sizeof(dev::TileOp));
+  dispatch([&](auto tag) {
+    using R = decltype(tag);
+    std::vector<amp_t<R>> m(mats.size());
+    for (size_t i = 0; i < mats.size(); ++i) {
+      m[i].x = (R)mats[i].real();
+      m[i].y = (R)mats[i].imag();
+    }
+    const amp_t<R> *dm = (const amp_t<R> *)upload(m.data(), m.size() * sizeof(amp_t<R>));
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (profiling_) {
+      e0 = take_event();
+      e1 = take_event();
+      QS_HIP(hipEventRecord(e0, stream_));
+    }
+    dev::tile_pass<R>((amp_t<R> *)psi_, n_tiles, P, d_ops, dm, stream_);
+    if (profiling_) {
+      QS_HIP(hipEventRecord(e1, stream_));
+      ev_used_.push_back({e0, e1});
+    }
+  });
+  count_pass(2.0 * (double)amp_bytes() * (double)amps());
 }
 
 void Engine::exec_block(const Block &b) {

@@ -15,6 +15,7 @@
 #include "common.hpp"
 #include "fusion.hpp"
 #include "kernels.hpp"
+#include "planner.hpp"
 
 namespace qs {
 
@@ -94,6 +95,9 @@ class Engine {
   void math_apply(dev::MathArgs args, const int32_t *d_table, bool known_bijective);
   double masked_norm(uint64_t mask, uint64_t val);
   void flush_window();
+  void flush_window_tiles(std::vector<Gate> &gates);
+  unsigned tile_bits() const;
+  void exec_tile_pass(const std::vector<PGate> &gates, const PPass &pass, unsigned t);
 
   bool c64_;
   int fusion_mode_, planner_kind_;

@@ -133,6 +133,107 @@ void launch_apply(amp_t<R> *psi, uint64_t n_groups, const BitInsert &ins, uint64
   QS_HIP(hipGetLastError());
 }
 
+// ------------------------------------------------------------------------------ tile passes
+template <typename R, int K>
+__device__ __forceinline__ void tile_apply(amp_t<R> *__restrict__ tile, uint32_t t,
+                                           const TileOp *__restrict__ op,
+                                           const amp_t<R> *__restrict__ m) {
+  constexpr int D = 1 << K;
+  const uint32_t nfixed = op->nfixed, lcmask = op->lcmask;
+  const uint32_t ngroups = 1u << (t - nfixed);
+  for (uint32_t g = threadIdx.x; g < ngroups; g += kBlock) {
+    uint32_t lb = g;
+    for (uint32_t i = 0; i < nfixed; ++i) {
+      const uint32_t p = op->fixed[i];
+      lb = ((lb >> p) << (p + 1)) | (lb & ((1u << p) - 1));
+    }
+    lb |= lcmask;
+    R vr[D], vi[D];
+#pragma unroll
+    for (int j = 0; j < D; ++j) {
+      amp_t<R> a = tile[lb + op->off[j]];
+      vr[j] = a.x;
+      vi[j] = a.y;
+    }
+#pragma unroll(K <= 3 ? D : 1)
+    for (int r = 0; r < D; ++r) {
+      R ar = 0, ai = 0;
+#pragma unroll
+      for (int c = 0; c < D; ++c) {
+        const amp_t<R> mc = m[r * D + c];
+        ar = fma(mc.x, vr[c], ar);
+        ar = fma(-mc.y, vi[c], ar);
+        ai = fma(mc.x, vi[c], ai);
+        ai = fma(mc.y, vr[c], ai);
+      }
+      amp_t<R> o;
+      o.x = ar;
+      o.y = ai;
+      tile[lb + op->off[r]] = o;
+    }
+  }
+}
+
+// 16-byte vectors of amplitudes used to stream tiles (2 x complex64 or 1 x complex128)
+template <typename R>
+struct Vec16 {
+  using type = float4;
+  static constexpr int n = 2;
+};
+template <>
+struct Vec16<double> {
+  using type = double2;
+  static constexpr int n = 1;
+};
+
+template <typename R>
+__global__ __launch_bounds__(kBlock) void k_tile_pass(amp_t<R> *__restrict__ psi, uint64_t n_tiles,
+                                                      TilePass P, const TileOp *__restrict__ ops,
+                                                      const amp_t<R> *__restrict__ mats) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  __shared__ uint64_t seg[1 << kTileMaxHigh];
+  using V = typename Vec16<R>::type;
+  constexpr int VN = Vec16<R>::n;
+  amp_t<R> *tile = reinterpret_cast<amp_t<R> *>(smem);
+  V *tilev = reinterpret_cast<V *>(smem);
+  const uint32_t t = P.t, L = P.L, h = P.h;
+  const uint32_t tsize = 1u << t, segmask = (1u << L) - 1;
+  for (uint32_t sgi = threadIdx.x; sgi < (1u << h); sgi += kBlock) {
+    uint64_t o = 0;
+    for (uint32_t b = 0; b < h; ++b)
+      if ((sgi >> b) & 1) o |= uint64_t(1) << P.hpos[b];
+    seg[sgi] = o;
+  }
+  __syncthreads();
+  for (uint64_t tix = blockIdx.x; tix < n_tiles; tix += gridDim.x) {
+    const uint64_t base = insert_zeros(tix, P.tiles);
+    for (uint32_t l = threadIdx.x * VN; l < tsize; l += kBlock * VN) {
+      const V *src = reinterpret_cast<const V *>(psi + (base | seg[l >> L] | (l & segmask)));
+      tilev[l / VN] = *src;
+    }
+    __syncthreads();
+    for (uint32_t o = 0; o < P.n_ops; ++o) {
+      const TileOp *op = ops + o;
+      const uint64_t gc = op->gcmask;
+      if ((base & gc) != gc) continue;  // uniform: every thread of the tile skips
+      const amp_t<R> *m = mats + op->mat;
+      switch (op->k) {
+        case 1: tile_apply<R, 1>(tile, t, op, m); break;
+        case 2: tile_apply<R, 2>(tile, t, op, m); break;
+        case 3: tile_apply<R, 3>(tile, t, op, m); break;
+        case 4: tile_apply<R, 4>(tile, t, op, m); break;
+        default: tile_apply<R, 5>(tile, t, op, m); break;
+      }
+      __syncthreads();
+    }
+    for (uint32_t l = threadIdx.x * VN; l < tsize; l += kBlock * VN) {
+      V *dst = reinterpret_cast<V *>(psi + (base | seg[l >> L] | (l & segmask)));
+      *dst = tilev[l / VN];
+    }
+    __syncthreads();
+  }
+}
+
 // ------------------------------------------------------------------------------ reductions
 template <typename R>
 __global__ __launch_bounds__(kBlock) void k_norm_masked(const amp_t<R> *__restrict__ psi,
@@ -577,6 +678,22 @@ void apply_block(amp_t<R> *psi, uint64_t n_groups, const BitInsert &ins, uint64_
 }
 
 template <typename R>
+void tile_pass(amp_t<R> *psi, uint64_t n_tiles, const TilePass &P, const TileOp *d_ops,
+               const amp_t<R> *d_mats, hipStream_t s) {
+  static bool attr_set = false;
+  const size_t lds = sizeof(amp_t<R>) << P.t;
+  if (!attr_set) {
+    QS_HIP(hipFuncSetAttribute((const void *)k_tile_pass<R>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                               (int)(sizeof(amp_t<R>) << 13)));
+    attr_set = true;
+  }
+  if (P.h > (uint32_t)kTileMaxHigh || P.L < 1 || P.t > 13) throw ValueError("tile pass: bad tile shape");
+  const uint64_t grid = std::min<uint64_t>(n_tiles, 1u << 20);
+  hipLaunchKernelGGL(k_tile_pass<R>, dim3((unsigned)grid), dim3(kBlock), lds, s, psi, n_tiles, P, d_ops, d_mats);
+  QS_HIP(hipGetLastError());
+}
+
+template <typename R>
 void norm_masked(const amp_t<R> *psi, uint64_t n_sub, const BitInsert &ins, uint64_t val,
                  double *partials, hipStream_t s) {
   hipLaunchKernelGGL(k_norm_masked<R>, dim3(kReduceGrid), dim3(kBlock), 0, s, psi, n_sub, ins, val,
@@ -711,6 +828,8 @@ void math_accumulate(const amp_t<R> *src, amp_t<R> *dst, uint64_t n, const MathA
 
 // ------------------------------------------------------------------------------ instantiations
 #define QS_INSTANTIATE(R)                                                                          \
+  template void tile_pass<R>(amp_t<R> *, uint64_t, const TilePass &, const TileOp *,              \
+                             const amp_t<R> *, hipStream_t);                                      \
   template void apply_block<R>(amp_t<R> *, uint64_t, const BitInsert &, uint64_t,                 \
                                const uint32_t *, int, const amp_t<R> *, hipStream_t);             \
   template void norm_masked<R>(const amp_t<R> *, uint64_t, const BitInsert &, uint64_t, double *, \

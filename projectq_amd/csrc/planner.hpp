@@ -1,0 +1,76 @@
+// Window planners: turn a queue of gates into device passes.
+//
+// The tile scheduler groups gates into tile passes.  A pass streams the state once; every tile
+// (2^t amplitudes sharing their non-tile bits) is staged in LDS and every gate of the pass whose
+// targets lie in the tile is applied there.  Gates only move past gates on disjoint qubits, so
+// the order of operations on every qubit is the queued order and results equal the unfused
+// sequence up to rounding.
+#pragma once
+
+#include <complex>
+#include <cstdint>
+#include <functional>
+#include <vector>
+
+#include "layout.hpp"
+
+namespace qs {
+
+// A queued gate resolved to bit positions (matrix bit l <-> tpos[l]).
+struct PGate {
+  std::vector<std::complex<double>> m;
+  std::vector<uint32_t> tpos;
+  uint64_t cmask = 0;
+  uint64_t tmask() const {
+    uint64_t v = 0;
+    for (uint32_t p : tpos) v |= uint64_t(1) << p;
+    return v;
+  }
+};
+
+// One tile pass: low tile bits 0..L-1 plus the `high` positions; gates in execution order.
+struct PPass {
+  uint32_t L = 0;
+  std::vector<uint32_t> high;
+  std::vector<int> gates;
+};
+
+struct TileShape {
+  unsigned n;     // qubits in the state
+  unsigned t;     // tile bits
+  unsigned Lmin;  // low bits every tile keeps (contiguous segment of 2^Lmin amplitudes)
+};
+
+// Description of a candidate tile choice that a cost model can score: for every high position
+// the planner could add, how much queued work it unlocks.
+struct SlotCandidate {
+  uint32_t pos;
+  double lookahead_score;  // decayed count of queued gates targeting pos
+  double first_rank;       // rank of the first queued gate targeting pos (0 = next gate)
+  double count;            // queued gates in the lookahead window that target pos
+};
+
+// Returns the high positions to add beyond the forced ones (at most `slots`), best first.
+using SlotChooser =
+    std::function<std::vector<uint32_t>(const std::vector<SlotCandidate> &cands, unsigned slots)>;
+
+// Merge runs of uncontrolled single-qubit gates on the same position (no other gate touching
+// that position in between) into one matrix product.
+std::vector<PGate> merge_single_qubit_runs(const std::vector<PGate> &gates);
+
+// Tile scheduling: repeatedly pick a tile (forced: the targets of the first unscheduled gate;
+// the other high slots from `choose`, default = highest lookahead score) and take, in queue
+// order, every unscheduled gate whose predecessors are scheduled and whose targets are in it.
+std::vector<PPass> schedule_tiles(const std::vector<PGate> &gates, const TileShape &shape,
+                                  const SlotChooser &choose = nullptr, unsigned lookahead = 256);
+
+// Device description of one tile pass: tile shape, one TileOp per gate (tile-local targets,
+// controls split into tile-local and outside-tile masks) and the matrices in op order.
+void build_tile_pass(const std::vector<PGate> &gates, const PPass &pass, unsigned t,
+                     dev::TilePass &P, std::vector<dev::TileOp> &ops,
+                     std::vector<std::complex<double>> &mats);
+
+// Default slot choice: highest lookahead score first.
+std::vector<uint32_t> choose_by_score(const std::vector<SlotCandidate> &cands, unsigned slots);
+
+}  // namespace qs

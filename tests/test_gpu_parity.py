@@ -79,14 +79,46 @@ def test_dense_controlled_blocks(k):
         assert np.max(np.abs(o.cheat_numpy()[1] - h.cheat_numpy()[1])) <= AMP_TOL
 
 
-@pytest.mark.parametrize("fusion", ["window"])
-def test_window_fusion_matches_unfused_oracle(fusion):
-    cmds = circuits.mixed_circuit(14, 10, 31)
+WINDOW_CIRCUITS = [
+    ("mixed14", lambda: circuits.mixed_circuit(14, 10, 31)),
+    ("random18", lambda: circuits.random_layers(18, 12, 9, 32, measure=False)),
+    ("random21", lambda: circuits.random_layers(21, 8, 10, 33, measure=False)),
+    ("qft16", lambda: circuits.allocate(16) + [commands.gate(G.H, [q]) for q in range(16)] + circuits.qft(list(range(16)))),
+    ("mixed9", lambda: circuits.mixed_circuit(9, 12, 34)),
+]
+
+
+@pytest.mark.parametrize("planner", ["greedy", "tile"])
+@pytest.mark.parametrize("name,make", WINDOW_CIRCUITS, ids=[c[0] for c in WINDOW_CIRCUITS])
+def test_window_planners_match_unfused_oracle(planner, name, make):
+    print(f"window planner={planner} circuit={name}", flush=True)
+    cmds = make()
     o = OracleSimulator(3)
     commands.execute(o, cmds, False)
-    h = Simulator(3, fusion=fusion)
+    h = Simulator(3, fusion="window", planner=planner)
     commands.execute(h, cmds, True)
     assert np.max(np.abs(o.cheat_numpy()[1] - h.cheat_numpy()[1])) <= AMP_TOL
+
+
+def test_tile_planner_controlled_blocks():
+    rng = np.random.default_rng(8)
+    n = 15
+    cmds = circuits.allocate(n)
+    for _ in range(80):
+        k = int(rng.integers(1, 6))
+        picks = rng.choice(n, size=k + int(rng.integers(0, 4)), replace=False)
+        a = rng.normal(size=(2**k, 2**k)) + 1j * rng.normal(size=(2**k, 2**k))
+        u, _ = np.linalg.qr(a)
+        cmds.append(commands.gate(u, [int(q) for q in picks[:k]], [int(q) for q in picks[k:]]))
+    o = OracleSimulator(3)
+    commands.execute(o, cmds, False)
+    for prec, tol in (("c128", AMP_TOL), ("c64", None)):
+        h = Simulator(3, precision=prec, fusion="window", planner="tile")
+        commands.execute(h, cmds, True)
+        if tol is not None:
+            assert np.max(np.abs(o.cheat_numpy()[1] - h.cheat_numpy()[1])) <= tol
+        else:
+            assert infidelity(o.cheat_numpy()[1], h.cheat_numpy()[1]) <= 1e-6
 
 
 def _superposition(sim, n, seed):

@@ -1,0 +1,128 @@
+"""CPU checks of the tile planner and the tile-pass index arithmetic.
+
+tests/native/tile_emu.cpp runs the product planner (projectq_amd/csrc/planner.cpp) and replays
+each pass with the kernel's index arithmetic on the CPU, bounds-checking every index.  The
+result must equal the unfused gate sequence run by the oracle (<= 1e-12), every gate must be
+scheduled exactly once, and no index may leave its buffer."""
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from oracle.cpu_sim import OracleSimulator
+from projectq_amd import circuits, commands
+from projectq_amd import gates as G
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+NATIVE = os.path.join(ROOT, "tests", "native")
+LIB = os.path.join(NATIVE, "_build", "libtile_emu.so")
+
+
+@pytest.fixture(scope="module")
+def emu():
+    os.makedirs(os.path.dirname(LIB), exist_ok=True)
+    srcs = [os.path.join(NATIVE, "tile_emu.cpp"), os.path.join(ROOT, "projectq_amd", "csrc", "planner.cpp")]
+    subprocess.run(
+        ["g++", "-O2", "-std=c++17", "-shared", "-fPIC", "-I", os.path.join(ROOT, "projectq_amd", "csrc"), *srcs, "-o", LIB],
+        check=True,
+    )
+    lib = ctypes.CDLL(LIB)
+    lib.emu_run.restype = ctypes.c_int
+    return lib
+
+
+def run_emu(lib, n, t, lmin, cmds, state):
+    ks, tpos, cm, mats = [], [], [], []
+    for c in cmds:
+        if c[0] != "gate":
+            continue
+        _, m, targets, controls = c
+        ks.append(len(targets))
+        tpos.extend(targets)
+        mask = 0
+        for q in controls:
+            mask |= 1 << q
+        cm.append(mask)
+        mats.append(np.asarray(m, dtype=np.complex128).reshape(-1))
+    k = np.asarray(ks, dtype=np.uint32)
+    tp = np.asarray(tpos, dtype=np.uint32)
+    cmask = np.asarray(cm, dtype=np.uint64)
+    mflat = np.ascontiguousarray(np.concatenate(mats)) if mats else np.zeros(1, dtype=np.complex128)
+    st = np.ascontiguousarray(state.astype(np.complex128))
+    npass = ctypes.c_int()
+    err = ctypes.create_string_buffer(256)
+    rc = lib.emu_run(
+        ctypes.c_uint(n),
+        ctypes.c_uint(t),
+        ctypes.c_uint(lmin),
+        ctypes.c_int(len(ks)),
+        k.ctypes.data_as(ctypes.c_void_p),
+        tp.ctypes.data_as(ctypes.c_void_p),
+        cmask.ctypes.data_as(ctypes.c_void_p),
+        mflat.ctypes.data_as(ctypes.c_void_p),
+        st.ctypes.data_as(ctypes.c_void_p),
+        ctypes.byref(npass),
+        err,
+        ctypes.c_int(256),
+    )
+    assert rc == 0, err.value.decode()
+    return st, npass.value
+
+
+def oracle_state(cmds, n):
+    o = OracleSimulator(1)
+    commands.execute(o, cmds, gate_fusion=False)
+    return o.cheat_numpy()[1]
+
+
+def start_state(n):
+    s = np.zeros(2**n, dtype=np.complex128)
+    s[0] = 1.0
+    return s
+
+
+CASES = [
+    ("random_layers", lambda n: circuits.random_layers(n, 6, n // 2, 3, measure=False)),
+    ("mixed", lambda n: circuits.mixed_circuit(n, 8, 5)),
+    ("qft", lambda n: circuits.allocate(n) + [commands.gate(G.H, [q]) for q in range(n)] + circuits.qft(list(range(n)))),
+]
+
+
+@pytest.mark.parametrize("name,make", CASES, ids=[c[0] for c in CASES])
+@pytest.mark.parametrize("n,t,lmin", [(8, 8, 5), (12, 10, 4), (14, 10, 5), (15, 12, 4), (16, 13, 5)])
+def test_tile_plan_matches_unfused(emu, name, make, n, t, lmin):
+    cmds = make(n)
+    want = oracle_state(cmds, n)
+    got, npass = run_emu(emu, n, t, lmin, cmds, start_state(n))
+    assert np.max(np.abs(got - want)) <= 1e-12
+    n_gates = commands.count_gates(cmds)
+    assert 1 <= npass <= n_gates
+
+
+def test_random_controlled_blocks(emu):
+    rng = np.random.default_rng(4)
+    n = 13
+    cmds = circuits.allocate(n)
+    for _ in range(60):
+        k = int(rng.integers(1, 6))
+        picks = rng.choice(n, size=k + int(rng.integers(0, 4)), replace=False)
+        a = rng.normal(size=(2**k, 2**k)) + 1j * rng.normal(size=(2**k, 2**k))
+        u, _ = np.linalg.qr(a)
+        cmds.append(commands.gate(u, [int(q) for q in picks[:k]], [int(q) for q in picks[k:]]))
+    want = oracle_state(cmds, n)
+    got, _ = run_emu(emu, n, 10, 4, cmds, start_state(n))
+    assert np.max(np.abs(got - want)) <= 1e-12
+
+
+def test_pass_count_random32_layer(emu):
+    """The planner packs a 32-qubit random-circuit layer (80 gates) into a handful of passes;
+    counted with a 20-qubit stand-in of the same shape (t=13, L=5) to stay small on the CPU."""
+    n = 20
+    cmds = circuits.random_layers(n, 4, 10, 9, measure=False)
+    got, npass = run_emu(emu, n, 13, 5, cmds, start_state(n))
+    want = oracle_state(cmds, n)
+    assert np.max(np.abs(got - want)) <= 1e-12
+    assert npass <= 4 * 3
