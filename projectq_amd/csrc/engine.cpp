@@ -390,28 +390,32 @@ void Engine::flush_window_tiles(std::vector<Gate> &gates) {
 }
 
 void Engine::exec_tile_pass(const std::vector<PGate> &gates, const PPass &ps, unsigned t) {
-  dev::TilePass P{};
-  std::vector<dev::TileOp> ops;
-  std::vector<cplx> mats;
-  build_tile_pass(gates, ps, t, P, ops, mats);
-  if (ops.empty()) return;
+  TileProgram prog;
+  const unsigned rb = c64_ ? (unsigned)dev::reg_bits<float>() : (unsigned)dev::reg_bits<double>();
+  build_tile_pass(gates, ps, t, rb, prog);
+  if (prog.stages.empty()) return;
   const uint64_t n_tiles = amps() >> t;
-  const dev::TileOp *d_ops = (const dev::TileOp *)upload(ops.data(), ops.size() * sizeof(dev::TileOp));
+  auto up = [&](const auto &v) -> const void * {
+    return v.empty() ? nullptr : upload(v.data(), v.size() * sizeof(v[0]));
+  };
+  const dev::TileStage *d_st = (const dev::TileStage *)up(prog.stages);
+  const dev::RegOp *d_rops = (const dev::RegOp *)up(prog.rops);
+  const dev::TileOp *d_tops = (const dev::TileOp *)up(prog.tops);
   dispatch([&](auto tag) {
     using R = decltype(tag);
-    std::vector<amp_t<R>> m(mats.size());
-    for (size_t i = 0; i < mats.size(); ++i) {
-      m[i].x = (R)mats[i].real();
-      m[i].y = (R)mats[i].imag();
+    std::vector<amp_t<R>> m(prog.mats.size());
+    for (size_t i = 0; i < m.size(); ++i) {
+      m[i].x = (R)prog.mats[i].real();
+      m[i].y = (R)prog.mats[i].imag();
     }
-    const amp_t<R> *dm = (const amp_t<R> *)upload(m.data(), m.size() * sizeof(amp_t<R>));
+    const amp_t<R> *dm = (const amp_t<R> *)up(m);
     hipEvent_t e0 = nullptr, e1 = nullptr;
     if (profiling_) {
       e0 = take_event();
       e1 = take_event();
       QS_HIP(hipEventRecord(e0, stream_));
     }
-    dev::tile_pass<R>((amp_t<R> *)psi_, n_tiles, P, d_ops, dm, stream_);
+    dev::tile_pass<R>((amp_t<R> *)psi_, n_tiles, prog.P, d_st, d_rops, d_tops, dm, stream_);
     if (profiling_) {
       QS_HIP(hipEventRecord(e1, stream_));
       ev_used_.push_back({e0, e1});

@@ -134,6 +134,18 @@ void launch_apply(amp_t<R> *psi, uint64_t n_groups, const BitInsert &ins, uint64
 }
 
 // ------------------------------------------------------------------------------ tile passes
+template <typename R>
+__device__ __forceinline__ uint32_t slot(uint32_t x);
+template <>
+__device__ __forceinline__ uint32_t slot<float>(uint32_t x) {
+  return lds_slot_c64(x);
+}
+template <>
+__device__ __forceinline__ uint32_t slot<double>(uint32_t x) {
+  return lds_slot_c128(x);
+}
+
+// One TileOp of any width straight on the LDS tile.
 template <typename R, int K>
 __device__ __forceinline__ void tile_apply(amp_t<R> *__restrict__ tile, uint32_t t,
                                            const TileOp *__restrict__ op,
@@ -151,7 +163,7 @@ __device__ __forceinline__ void tile_apply(amp_t<R> *__restrict__ tile, uint32_t
     R vr[D], vi[D];
 #pragma unroll
     for (int j = 0; j < D; ++j) {
-      amp_t<R> a = tile[lb + op->off[j]];
+      amp_t<R> a = tile[slot<R>(lb + op->off[j])];
       vr[j] = a.x;
       vi[j] = a.y;
     }
@@ -169,7 +181,193 @@ __device__ __forceinline__ void tile_apply(amp_t<R> *__restrict__ tile, uint32_t
       amp_t<R> o;
       o.x = ar;
       o.y = ai;
-      tile[lb + op->off[r]] = o;
+      tile[slot<R>(lb + op->off[r])] = o;
+    }
+  }
+}
+
+// ---- register-stage ops: NR = 2^RB amplitudes of one thread, register index bit i <-> stage
+// bit i; rc = controls among register bits (a compile-time index i passes iff (i & rc) == rc).
+template <typename R, int RB, int B>
+__device__ __forceinline__ void rop_general(R *vr, R *vi, uint32_t rc, const amp_t<R> *__restrict__ m) {
+  const R ar = m[0].x, ai = m[0].y, br = m[1].x, bi = m[1].y;
+  const R cr = m[2].x, ci = m[2].y, dr = m[3].x, di = m[3].y;
+#pragma unroll
+  for (int i = 0; i < (1 << RB); ++i) {
+    if (i & (1 << B)) continue;
+    if ((i & rc) != rc) continue;
+    const int j = i | (1 << B);
+    const R xr = vr[i], xi = vi[i], yr = vr[j], yi = vi[j];
+    vr[i] = fma(ar, xr, fma(-ai, xi, fma(br, yr, -bi * yi)));
+    vi[i] = fma(ar, xi, fma(ai, xr, fma(br, yi, bi * yr)));
+    vr[j] = fma(cr, xr, fma(-ci, xi, fma(dr, yr, -di * yi)));
+    vi[j] = fma(cr, xi, fma(ci, xr, fma(dr, yi, di * yr)));
+  }
+}
+template <typename R, int RB, int B>
+__device__ __forceinline__ void rop_diag(R *vr, R *vi, uint32_t rc, const amp_t<R> *__restrict__ m) {
+  const R ar = m[0].x, ai = m[0].y, dr = m[3].x, di = m[3].y;
+#pragma unroll
+  for (int i = 0; i < (1 << RB); ++i) {
+    if ((i & rc) != rc) continue;
+    const R fr = (i & (1 << B)) ? dr : ar, fi = (i & (1 << B)) ? di : ai;
+    const R xr = vr[i], xi = vi[i];
+    vr[i] = fma(fr, xr, -fi * xi);
+    vi[i] = fma(fr, xi, fi * xr);
+  }
+}
+template <typename R, int RB, int B>
+__device__ __forceinline__ void rop_anti(R *vr, R *vi, uint32_t rc, const amp_t<R> *__restrict__ m) {
+  const R br = m[1].x, bi = m[1].y, cr = m[2].x, ci = m[2].y;
+#pragma unroll
+  for (int i = 0; i < (1 << RB); ++i) {
+    if (i & (1 << B)) continue;
+    if ((i & rc) != rc) continue;
+    const int j = i | (1 << B);
+    const R xr = vr[i], xi = vi[i], yr = vr[j], yi = vi[j];
+    vr[i] = fma(br, yr, -bi * yi);
+    vi[i] = fma(br, yi, bi * yr);
+    vr[j] = fma(cr, xr, -ci * xi);
+    vi[j] = fma(cr, xi, ci * xr);
+  }
+}
+template <typename R, int RB, int B>
+__device__ __forceinline__ void rop_x(R *vr, R *vi, uint32_t rc, const amp_t<R> *__restrict__) {
+#pragma unroll
+  for (int i = 0; i < (1 << RB); ++i) {
+    if (i & (1 << B)) continue;
+    if ((i & rc) != rc) continue;
+    const int j = i | (1 << B);
+    const R xr = vr[i], xi = vi[i];
+    vr[i] = vr[j];
+    vi[i] = vi[j];
+    vr[j] = xr;
+    vi[j] = xi;
+  }
+}
+template <typename R, int RB, int B0, int B1>
+__device__ __forceinline__ void rop_general2(R *vr, R *vi, uint32_t rc, const amp_t<R> *__restrict__ m) {
+#pragma unroll
+  for (int i = 0; i < (1 << RB); ++i) {
+    if (i & ((1 << B0) | (1 << B1))) continue;
+    if ((i & rc) != rc) continue;
+    const int idx[4] = {i, i | (1 << B0), i | (1 << B1), i | (1 << B0) | (1 << B1)};
+    R xr[4], xi[4];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      xr[c] = vr[idx[c]];
+      xi[c] = vi[idx[c]];
+    }
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      R sr = 0, si = 0;
+#pragma unroll
+      for (int c = 0; c < 4; ++c) {
+        const amp_t<R> mc = m[r * 4 + c];
+        sr = fma(mc.x, xr[c], fma(-mc.y, xi[c], sr));
+        si = fma(mc.x, xi[c], fma(mc.y, xr[c], si));
+      }
+      vr[idx[r]] = sr;
+      vi[idx[r]] = si;
+    }
+  }
+}
+template <typename R, int RB, int B0, int B1>
+__device__ __forceinline__ void rop_diag2(R *vr, R *vi, uint32_t rc, const amp_t<R> *__restrict__ m) {
+#pragma unroll
+  for (int i = 0; i < (1 << RB); ++i) {
+    if ((i & rc) != rc) continue;
+    const int s = ((i >> B0) & 1) | (((i >> B1) & 1) << 1);
+    const amp_t<R> f = m[s * 5];  // diagonal element s of the 4x4 matrix
+    const R xr = vr[i], xi = vi[i];
+    vr[i] = fma(f.x, xr, -f.y * xi);
+    vi[i] = fma(f.x, xi, f.y * xr);
+  }
+}
+
+#define QS_ROP1(FN)                                                                     \
+  switch (b0) {                                                                         \
+    case 0: FN<R, RB, 0>(vr, vi, rc, m); break;                                         \
+    case 1: FN<R, RB, 1>(vr, vi, rc, m); break;                                         \
+    case 2: FN<R, RB, 2>(vr, vi, rc, m); break;                                         \
+    case 3: FN<R, RB, 3>(vr, vi, rc, m); break;                                         \
+    default:                                                                            \
+      if constexpr (RB > 4) FN<R, RB, 4>(vr, vi, rc, m);                                \
+      break;                                                                            \
+  }
+#define QS_ROP2(FN)                                                                     \
+  switch (b0 * 8 + b1) {                                                                \
+    case 0 * 8 + 1: FN<R, RB, 0, 1>(vr, vi, rc, m); break;                              \
+    case 0 * 8 + 2: FN<R, RB, 0, 2>(vr, vi, rc, m); break;                              \
+    case 0 * 8 + 3: FN<R, RB, 0, 3>(vr, vi, rc, m); break;                              \
+    case 1 * 8 + 2: FN<R, RB, 1, 2>(vr, vi, rc, m); break;                              \
+    case 1 * 8 + 3: FN<R, RB, 1, 3>(vr, vi, rc, m); break;                              \
+    case 2 * 8 + 3: FN<R, RB, 2, 3>(vr, vi, rc, m); break;                              \
+    case 0 * 8 + 4: if constexpr (RB > 4) FN<R, RB, 0, 4>(vr, vi, rc, m); break;        \
+    case 1 * 8 + 4: if constexpr (RB > 4) FN<R, RB, 1, 4>(vr, vi, rc, m); break;        \
+    case 2 * 8 + 4: if constexpr (RB > 4) FN<R, RB, 2, 4>(vr, vi, rc, m); break;        \
+    default: if constexpr (RB > 4) FN<R, RB, 3, 4>(vr, vi, rc, m); break;               \
+  }
+
+template <typename R, int RB>
+__device__ __forceinline__ void reg_op(uint32_t code, R *vr, R *vi, uint32_t rc,
+                                       const amp_t<R> *__restrict__ m) {
+  const uint32_t kind = code >> 8, b0 = code & 15u, b1 = (code >> 4) & 15u;
+  switch (kind) {
+    case kOp1General: QS_ROP1(rop_general) break;
+    case kOp1Diag: QS_ROP1(rop_diag) break;
+    case kOp1Anti: QS_ROP1(rop_anti) break;
+    case kOp1X: QS_ROP1(rop_x) break;
+    case kOp2General: QS_ROP2(rop_general2) break;
+    default: QS_ROP2(rop_diag2) break;
+  }
+}
+
+template <typename R, int RB>
+__device__ __forceinline__ void reg_stage(amp_t<R> *__restrict__ tile, uint32_t t, uint64_t base,
+                                          const TileStage *__restrict__ st,
+                                          const RegOp *__restrict__ rops,
+                                          const amp_t<R> *__restrict__ mats) {
+  constexpr int NR = 1 << RB;
+  uint32_t roff[NR];
+#pragma unroll
+  for (int i = 0; i < NR; ++i) {
+    uint32_t o = 0;
+#pragma unroll
+    for (int b = 0; b < RB; ++b)
+      if ((i >> b) & 1) o |= 1u << st->fixed[b];
+    roff[i] = o;
+  }
+  const uint32_t ngroups = 1u << (t - RB);
+  const uint32_t ob = st->begin, oe = st->end;
+  for (uint32_t g = threadIdx.x; g < ngroups; g += kBlock) {
+    uint32_t lb = g;
+#pragma unroll
+    for (int b = 0; b < RB; ++b) {
+      const uint32_t p = st->fixed[b];
+      lb = ((lb >> p) << (p + 1)) | (lb & ((1u << p) - 1));
+    }
+    R vr[NR], vi[NR];
+#pragma unroll
+    for (int i = 0; i < NR; ++i) {
+      const amp_t<R> a = tile[slot<R>(lb | roff[i])];
+      vr[i] = a.x;
+      vi[i] = a.y;
+    }
+    for (uint32_t o = ob; o < oe; ++o) {
+      const RegOp *op = rops + o;
+      const uint64_t gc = op->gcmask;
+      if ((base & gc) != gc) continue;
+      const uint32_t lc = op->lcmask;
+      if ((lb & lc) != lc) continue;
+      reg_op<R, RB>(op->code, vr, vi, op->rcmask, mats + op->mat);
+    }
+#pragma unroll
+    for (int i = 0; i < NR; ++i) {
+      amp_t<R> a;
+      a.x = vr[i];
+      a.y = vi[i];
+      tile[slot<R>(lb | roff[i])] = a;
     }
   }
 }
@@ -186,16 +384,17 @@ struct Vec16<double> {
   static constexpr int n = 1;
 };
 
-template <typename R>
+template <typename R, int RB>
 __global__ __launch_bounds__(kBlock) void k_tile_pass(amp_t<R> *__restrict__ psi, uint64_t n_tiles,
-                                                      TilePass P, const TileOp *__restrict__ ops,
+                                                      TilePass P, const TileStage *__restrict__ stages,
+                                                      const RegOp *__restrict__ rops,
+                                                      const TileOp *__restrict__ tops,
                                                       const amp_t<R> *__restrict__ mats) {
   extern __shared__ __align__(16) unsigned char smem[];
   __shared__ uint64_t seg[1 << kTileMaxHigh];
   using V = typename Vec16<R>::type;
   constexpr int VN = Vec16<R>::n;
   amp_t<R> *tile = reinterpret_cast<amp_t<R> *>(smem);
-  V *tilev = reinterpret_cast<V *>(smem);
   const uint32_t t = P.t, L = P.L, h = P.h;
   const uint32_t tsize = 1u << t, segmask = (1u << L) - 1;
   for (uint32_t sgi = threadIdx.x; sgi < (1u << h); sgi += kBlock) {
@@ -209,26 +408,32 @@ __global__ __launch_bounds__(kBlock) void k_tile_pass(amp_t<R> *__restrict__ psi
     const uint64_t base = insert_zeros(tix, P.tiles);
     for (uint32_t l = threadIdx.x * VN; l < tsize; l += kBlock * VN) {
       const V *src = reinterpret_cast<const V *>(psi + (base | seg[l >> L] | (l & segmask)));
-      tilev[l / VN] = *src;
+      *reinterpret_cast<V *>(tile + slot<R>(l)) = *src;
     }
     __syncthreads();
-    for (uint32_t o = 0; o < P.n_ops; ++o) {
-      const TileOp *op = ops + o;
-      const uint64_t gc = op->gcmask;
-      if ((base & gc) != gc) continue;  // uniform: every thread of the tile skips
-      const amp_t<R> *m = mats + op->mat;
-      switch (op->k) {
-        case 1: tile_apply<R, 1>(tile, t, op, m); break;
-        case 2: tile_apply<R, 2>(tile, t, op, m); break;
-        case 3: tile_apply<R, 3>(tile, t, op, m); break;
-        case 4: tile_apply<R, 4>(tile, t, op, m); break;
-        default: tile_apply<R, 5>(tile, t, op, m); break;
+    for (uint32_t s = 0; s < P.n_stages; ++s) {
+      const TileStage *st = stages + s;
+      if (st->kind == kStageRegisters) {
+        reg_stage<R, RB>(tile, t, base, st, rops, mats);
+      } else {
+        const TileOp *op = tops + st->begin;
+        const uint64_t gc = op->gcmask;
+        if ((base & gc) == gc) {
+          const amp_t<R> *m = mats + op->mat;
+          switch (op->k) {
+            case 1: tile_apply<R, 1>(tile, t, op, m); break;
+            case 2: tile_apply<R, 2>(tile, t, op, m); break;
+            case 3: tile_apply<R, 3>(tile, t, op, m); break;
+            case 4: tile_apply<R, 4>(tile, t, op, m); break;
+            default: tile_apply<R, 5>(tile, t, op, m); break;
+          }
+        }
       }
       __syncthreads();
     }
     for (uint32_t l = threadIdx.x * VN; l < tsize; l += kBlock * VN) {
       V *dst = reinterpret_cast<V *>(psi + (base | seg[l >> L] | (l & segmask)));
-      *dst = tilev[l / VN];
+      *dst = *reinterpret_cast<const V *>(tile + slot<R>(l));
     }
     __syncthreads();
   }
@@ -678,18 +883,21 @@ void apply_block(amp_t<R> *psi, uint64_t n_groups, const BitInsert &ins, uint64_
 }
 
 template <typename R>
-void tile_pass(amp_t<R> *psi, uint64_t n_tiles, const TilePass &P, const TileOp *d_ops,
-               const amp_t<R> *d_mats, hipStream_t s) {
+void tile_pass(amp_t<R> *psi, uint64_t n_tiles, const TilePass &P, const TileStage *d_stages,
+               const RegOp *d_rops, const TileOp *d_tops, const amp_t<R> *d_mats, hipStream_t s) {
+  constexpr int RB = reg_bits<R>();
   static bool attr_set = false;
   const size_t lds = sizeof(amp_t<R>) << P.t;
   if (!attr_set) {
-    QS_HIP(hipFuncSetAttribute((const void *)k_tile_pass<R>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                               (int)(sizeof(amp_t<R>) << 13)));
+    QS_HIP(hipFuncSetAttribute((const void *)k_tile_pass<R, RB>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                               (int)(sizeof(amp_t<R>) << kTileMaxBits)));
     attr_set = true;
   }
-  if (P.h > (uint32_t)kTileMaxHigh || P.L < 1 || P.t > 13) throw ValueError("tile pass: bad tile shape");
+  if (P.h > (uint32_t)kTileMaxHigh || P.L < 1 || P.t > (uint32_t)kTileMaxBits)
+    throw ValueError("tile pass: bad tile shape");
   const uint64_t grid = std::min<uint64_t>(n_tiles, 1u << 20);
-  hipLaunchKernelGGL(k_tile_pass<R>, dim3((unsigned)grid), dim3(kBlock), lds, s, psi, n_tiles, P, d_ops, d_mats);
+  hipLaunchKernelGGL((k_tile_pass<R, RB>), dim3((unsigned)grid), dim3(kBlock), lds, s, psi, n_tiles, P,
+                     d_stages, d_rops, d_tops, d_mats);
   QS_HIP(hipGetLastError());
 }
 
@@ -828,8 +1036,8 @@ void math_accumulate(const amp_t<R> *src, amp_t<R> *dst, uint64_t n, const MathA
 
 // ------------------------------------------------------------------------------ instantiations
 #define QS_INSTANTIATE(R)                                                                          \
-  template void tile_pass<R>(amp_t<R> *, uint64_t, const TilePass &, const TileOp *,              \
-                             const amp_t<R> *, hipStream_t);                                      \
+  template void tile_pass<R>(amp_t<R> *, uint64_t, const TilePass &, const TileStage *,           \
+                             const RegOp *, const TileOp *, const amp_t<R> *, hipStream_t);       \
   template void apply_block<R>(amp_t<R> *, uint64_t, const BitInsert &, uint64_t,                 \
                                const uint32_t *, int, const amp_t<R> *, hipStream_t);             \
   template void norm_masked<R>(const amp_t<R> *, uint64_t, const BitInsert &, uint64_t, double *, \

@@ -27,10 +27,14 @@ struct MathArgs {
   uint8_t pos[kMaxRegs][32];
 };
 
-// tile passes (TileOp / TilePass in layout.hpp)
+// tile passes (TilePass / TileStage / RegOp / TileOp in layout.hpp)
 template <typename R>
-void tile_pass(amp_t<R> *psi, uint64_t n_tiles, const TilePass &P, const TileOp *d_ops,
-               const amp_t<R> *d_mats, hipStream_t s);
+constexpr int reg_bits() {
+  return sizeof(R) == 4 ? 5 : 4;
+}
+template <typename R>
+void tile_pass(amp_t<R> *psi, uint64_t n_tiles, const TilePass &P, const TileStage *d_stages,
+               const RegOp *d_rops, const TileOp *d_tops, const amp_t<R> *d_mats, hipStream_t s);
 
 // dense gate block: psi[base + off(r)] = sum_c M[r][c] psi[base + off(c)] for every base with
 // target bits 0 and control bits 1.  tpos[l] is the position of matrix bit l; ins lists the

@@ -136,9 +136,121 @@ std::vector<PPass> schedule_tiles(const std::vector<PGate> &gates, const TileSha
   return passes;
 }
 
-void build_tile_pass(const std::vector<PGate> &gates, const PPass &ps, unsigned t, dev::TilePass &P,
-                     std::vector<dev::TileOp> &ops, std::vector<std::complex<double>> &mats) {
-  P = dev::TilePass{};
+namespace {
+
+using cplx = std::complex<double>;
+
+uint32_t popcount32(uint32_t x) {
+  uint32_t c = 0;
+  for (; x; x &= x - 1) ++c;
+  return c;
+}
+
+// a gate of the pass in tile-local terms
+struct LocalGate {
+  uint32_t k;
+  uint32_t tb[5];
+  uint32_t lc;  // tile-local controls
+  uint64_t gc;  // controls outside the tile
+  const std::vector<cplx> *m;
+};
+
+uint32_t classify1(const std::vector<cplx> &m) {
+  const cplx zero(0.0, 0.0), one(1.0, 0.0);
+  if (m[0] == zero && m[3] == zero) return (m[1] == one && m[2] == one) ? dev::kOp1X : dev::kOp1Anti;
+  if (m[1] == zero && m[2] == zero) return dev::kOp1Diag;
+  return dev::kOp1General;
+}
+
+bool is_diag4(const std::vector<cplx> &m) {
+  for (int r = 0; r < 4; ++r)
+    for (int c = 0; c < 4; ++c)
+      if (r != c && m[r * 4 + c] != cplx(0.0, 0.0)) return false;
+  return true;
+}
+
+void emit_lds_op(const LocalGate &g, unsigned t, TileProgram &prog) {
+  dev::TileOp op{};
+  op.k = g.k;
+  std::vector<uint32_t> fixed(g.tb, g.tb + g.k);
+  op.lcmask = g.lc;
+  op.gcmask = g.gc;
+  for (uint32_t p = 0; p < 32; ++p)
+    if ((g.lc >> p) & 1) fixed.push_back(p);
+  std::sort(fixed.begin(), fixed.end());
+  if (std::adjacent_find(fixed.begin(), fixed.end()) != fixed.end() || fixed.size() > 16 || fixed.size() > t)
+    throw std::runtime_error("tile pass: repeated or too many fixed positions in one gate");
+  op.nfixed = (uint32_t)fixed.size();
+  for (size_t i = 0; i < fixed.size(); ++i) op.fixed[i] = fixed[i];
+  for (uint32_t j = 0; j < (1u << g.k); ++j) {
+    uint32_t o = 0;
+    for (uint32_t l = 0; l < g.k; ++l)
+      if ((j >> l) & 1) o |= 1u << g.tb[l];
+    op.off[j] = o;
+  }
+  op.mat = (uint32_t)prog.mats.size();
+  prog.mats.insert(prog.mats.end(), g.m->begin(), g.m->end());
+  dev::TileStage st{};
+  st.kind = dev::kStageLds;
+  st.begin = (uint32_t)prog.tops.size();
+  st.end = st.begin + 1;
+  prog.tops.push_back(op);
+  prog.stages.push_back(st);
+}
+
+void emit_reg_stage(const std::vector<LocalGate> &run, uint32_t bits, unsigned t, unsigned rb, TileProgram &prog) {
+  // pad the stage's register bits up to rb with the lowest other tile bits
+  for (uint32_t p = 0; p < t && popcount32(bits) < rb; ++p) bits |= 1u << p;
+  dev::TileStage st{};
+  st.kind = dev::kStageRegisters;
+  int regbit[32];
+  for (int &x : regbit) x = -1;
+  uint32_t nb = 0;
+  for (uint32_t p = 0; p < t; ++p)
+    if ((bits >> p) & 1) {
+      regbit[p] = (int)nb;
+      st.fixed[nb++] = p;
+    }
+  if (nb != rb) throw std::runtime_error("tile pass: register stage width");
+  st.begin = (uint32_t)prog.rops.size();
+  for (const LocalGate &g : run) {
+    dev::RegOp op{};
+    op.gcmask = g.gc;
+    for (uint32_t p = 0; p < 32; ++p) {
+      if (!((g.lc >> p) & 1)) continue;
+      if (regbit[p] >= 0) op.rcmask |= 1u << regbit[p];
+      else op.lcmask |= 1u << p;
+    }
+    op.mat = (uint32_t)prog.mats.size();
+    if (g.k == 1) {
+      const uint32_t b0 = (uint32_t)regbit[g.tb[0]];
+      op.code = classify1(*g.m) << 8 | b0;
+      prog.mats.insert(prog.mats.end(), g.m->begin(), g.m->end());
+    } else {
+      uint32_t b0 = (uint32_t)regbit[g.tb[0]], b1 = (uint32_t)regbit[g.tb[1]];
+      std::vector<cplx> m = *g.m;
+      if (b0 > b1) {  // make matrix bit 0 the lower register bit: swap index bits 0 and 1
+        auto sw = [](int x) { return ((x & 1) << 1) | ((x >> 1) & 1); };
+        std::vector<cplx> w(16);
+        for (int r = 0; r < 4; ++r)
+          for (int c = 0; c < 4; ++c) w[r * 4 + c] = m[sw(r) * 4 + sw(c)];
+        m.swap(w);
+        std::swap(b0, b1);
+      }
+      op.code = (is_diag4(m) ? dev::kOp2Diag : dev::kOp2General) << 8 | b1 << 4 | b0;
+      prog.mats.insert(prog.mats.end(), m.begin(), m.end());
+    }
+    prog.rops.push_back(op);
+  }
+  st.end = (uint32_t)prog.rops.size();
+  prog.stages.push_back(st);
+}
+
+}  // namespace
+
+void build_tile_pass(const std::vector<PGate> &gates, const PPass &ps, unsigned t, unsigned rb, TileProgram &prog) {
+  prog = TileProgram{};
+  dev::TilePass &P = prog.P;
   P.t = t;
   P.L = ps.L;
   P.h = (uint32_t)ps.high.size();
@@ -159,45 +271,44 @@ void build_tile_pass(const std::vector<PGate> &gates, const PPass &ps, unsigned 
   std::sort(tilepos.begin(), tilepos.end());
   P.tiles.count = (uint32_t)tilepos.size();
   for (size_t i = 0; i < tilepos.size(); ++i) P.tiles.pos[i] = (uint8_t)tilepos[i];
-  ops.clear();
-  mats.clear();
+
+  std::vector<LocalGate> run;
+  uint32_t run_bits = 0;
+  auto flush = [&]() {
+    if (!run.empty()) emit_reg_stage(run, run_bits, t, rb, prog);
+    run.clear();
+    run_bits = 0;
+  };
   for (int gi : ps.gates) {
     const PGate &g = gates[gi];
-    dev::TileOp op{};
-    op.k = (uint32_t)g.tpos.size();
-    if (op.k < 1 || op.k > 5) throw std::runtime_error("tile pass: gate width outside 1..5");
-    uint32_t tb[5];
-    std::vector<uint32_t> fixed;
-    for (uint32_t l = 0; l < op.k; ++l) {
+    LocalGate lg{};
+    lg.k = (uint32_t)g.tpos.size();
+    if (lg.k < 1 || lg.k > 5) throw std::runtime_error("tile pass: gate width outside 1..5");
+    uint32_t need = 0;
+    for (uint32_t l = 0; l < lg.k; ++l) {
       if (g.tpos[l] >= 64 || loc[g.tpos[l]] < 0) throw std::runtime_error("tile planner placed a gate outside its tile");
-      tb[l] = (uint32_t)loc[g.tpos[l]];
-      fixed.push_back(tb[l]);
+      lg.tb[l] = (uint32_t)loc[g.tpos[l]];
+      need |= 1u << lg.tb[l];
     }
+    if (popcount32(need) != lg.k) throw std::runtime_error("tile pass: repeated target");
     for (uint32_t p = 0; p < 64; ++p) {
       if (!((g.cmask >> p) & 1)) continue;
-      if (loc[p] >= 0) {
-        op.lcmask |= 1u << loc[p];
-        fixed.push_back((uint32_t)loc[p]);
-      } else {
-        op.gcmask |= uint64_t(1) << p;
-      }
+      if (loc[p] >= 0) lg.lc |= 1u << loc[p];
+      else lg.gc |= uint64_t(1) << p;
     }
-    std::sort(fixed.begin(), fixed.end());
-    if (std::adjacent_find(fixed.begin(), fixed.end()) != fixed.end() || fixed.size() > 16 || fixed.size() > t)
-      throw std::runtime_error("tile pass: repeated or too many fixed positions in one gate");
-    op.nfixed = (uint32_t)fixed.size();
-    for (size_t i = 0; i < fixed.size(); ++i) op.fixed[i] = fixed[i];
-    for (uint32_t j = 0; j < (1u << op.k); ++j) {
-      uint32_t o = 0;
-      for (uint32_t l = 0; l < op.k; ++l)
-        if ((j >> l) & 1) o |= 1u << tb[l];
-      op.off[j] = o;
+    lg.m = &g.m;
+    if (lg.k >= 3 || t < rb) {
+      flush();
+      emit_lds_op(lg, t, prog);
+      continue;
     }
-    op.mat = (uint32_t)mats.size();
-    mats.insert(mats.end(), g.m.begin(), g.m.end());
-    ops.push_back(op);
+    if (popcount32(run_bits | need) > rb) flush();
+    run_bits |= need;
+    run.push_back(lg);
   }
-  P.n_ops = (uint32_t)ops.size();
+  flush();
+  P.n_ops = (uint32_t)(prog.rops.size() + prog.tops.size());
+  P.n_stages = (uint32_t)prog.stages.size();
 }
 
 }  // namespace qs

@@ -64,11 +64,19 @@ std::vector<PGate> merge_single_qubit_runs(const std::vector<PGate> &gates);
 std::vector<PPass> schedule_tiles(const std::vector<PGate> &gates, const TileShape &shape,
                                   const SlotChooser &choose = nullptr, unsigned lookahead = 256);
 
-// Device description of one tile pass: tile shape, one TileOp per gate (tile-local targets,
-// controls split into tile-local and outside-tile masks) and the matrices in op order.
-void build_tile_pass(const std::vector<PGate> &gates, const PPass &pass, unsigned t,
-                     dev::TilePass &P, std::vector<dev::TileOp> &ops,
-                     std::vector<std::complex<double>> &mats);
+// Device description of one tile pass.  Gates are split into stages: runs of 1- and 2-qubit
+// gates whose targets fit `rb` register bits become one register stage (RegOps, specialised
+// by matrix shape: general, diagonal, anti-diagonal, X); wider gates become LDS stages
+// (TileOps).  Controls are split into register-bit, tile-local and outside-tile masks.
+struct TileProgram {
+  dev::TilePass P{};
+  std::vector<dev::TileStage> stages;
+  std::vector<dev::RegOp> rops;
+  std::vector<dev::TileOp> tops;
+  std::vector<std::complex<double>> mats;
+};
+void build_tile_pass(const std::vector<PGate> &gates, const PPass &pass, unsigned t, unsigned rb,
+                     TileProgram &prog);
 
 // Default slot choice: highest lookahead score first.
 std::vector<uint32_t> choose_by_score(const std::vector<SlotCandidate> &cands, unsigned slots);

@@ -32,12 +32,61 @@ void check(bool ok, const char *what) {
   if (!ok) throw Fail{what};
 }
 
-void replay(cplx *psi, unsigned n, const qs::dev::TilePass &P, const std::vector<qs::dev::TileOp> &ops,
-            const std::vector<cplx> &mats) {
+uint32_t slot(uint32_t x, bool c64) { return c64 ? qs::dev::lds_slot_c64(x) : qs::dev::lds_slot_c128(x); }
+
+void apply_small(std::vector<cplx> &v, uint32_t code, uint32_t rc, const cplx *m, unsigned rb) {
+  const uint32_t kind = code >> 8, b0 = code & 15u, b1 = (code >> 4) & 15u;
+  const uint32_t NR = 1u << rb;
+  check(b0 < rb && (kind < 4 || (b1 < rb && b0 < b1)), "register op bits");
+  if (kind < 4) {
+    for (uint32_t i = 0; i < NR; ++i) {
+      if ((i & rc) != rc) continue;
+      if (kind == qs::dev::kOp1Diag) {
+        v[i] *= ((i >> b0) & 1) ? m[3] : m[0];
+        continue;
+      }
+      if ((i >> b0) & 1) continue;
+      const uint32_t j = i | (1u << b0);
+      const cplx x = v[i], y = v[j];
+      if (kind == qs::dev::kOp1General) {
+        v[i] = m[0] * x + m[1] * y;
+        v[j] = m[2] * x + m[3] * y;
+      } else if (kind == qs::dev::kOp1Anti) {
+        v[i] = m[1] * y;
+        v[j] = m[2] * x;
+      } else {
+        v[i] = y;
+        v[j] = x;
+      }
+    }
+    return;
+  }
+  for (uint32_t i = 0; i < NR; ++i) {
+    if ((i & rc) != rc) continue;
+    if (kind == qs::dev::kOp2Diag) {
+      uint32_t s = ((i >> b0) & 1) | (((i >> b1) & 1) << 1);
+      v[i] *= m[s * 5];
+      continue;
+    }
+    if ((i >> b0) & 1 || (i >> b1) & 1) continue;
+    const uint32_t idx[4] = {i, i | (1u << b0), i | (1u << b1), i | (1u << b0) | (1u << b1)};
+    cplx x[4], y[4];
+    for (int c = 0; c < 4; ++c) x[c] = v[idx[c]];
+    for (int r = 0; r < 4; ++r) {
+      y[r] = 0.0;
+      for (int c = 0; c < 4; ++c) y[r] += m[r * 4 + c] * x[c];
+    }
+    for (int r = 0; r < 4; ++r) v[idx[r]] = y[r];
+  }
+}
+
+void replay(cplx *psi, unsigned n, const qs::TileProgram &prog, unsigned rb, bool c64) {
+  const qs::dev::TilePass &P = prog.P;
   const uint32_t t = P.t, L = P.L, h = P.h;
   const uint64_t N = uint64_t(1) << n;
   const uint32_t tsize = 1u << t, segmask = (1u << L) - 1;
   check(t <= n && L + h == t && h <= (uint32_t)qs::dev::kTileMaxHigh, "tile shape");
+  check(P.n_stages == prog.stages.size(), "stage count");
   std::vector<uint64_t> seg(1u << h);
   for (uint32_t s = 0; s < (1u << h); ++s) {
     uint64_t o = 0;
@@ -47,6 +96,12 @@ void replay(cplx *psi, unsigned n, const qs::dev::TilePass &P, const std::vector
   }
   const uint64_t n_tiles = N >> t;
   std::vector<cplx> tile(tsize);
+  auto lds = [&](uint32_t x) -> cplx & {
+    check(x < tsize, "LDS index");
+    uint32_t y = slot(x, c64);
+    check(y < tsize, "LDS slot");
+    return tile[y];
+  };
   for (uint64_t tix = 0; tix < n_tiles; ++tix) {
     const uint64_t base = insert_zeros(tix, P.tiles);
     check(base < N, "tile base");
@@ -54,34 +109,61 @@ void replay(cplx *psi, unsigned n, const qs::dev::TilePass &P, const std::vector
       check((l >> L) < seg.size(), "segment index");
       uint64_t gi = base | seg[l >> L] | (l & segmask);
       check(gi < N, "global index");
-      tile[l] = psi[gi];
+      lds(l) = psi[gi];
     }
-    for (const auto &op : ops) {
-      if ((base & op.gcmask) != op.gcmask) continue;
-      const uint32_t D = 1u << op.k;
-      check(op.k >= 1 && op.k <= 5 && op.nfixed <= t, "op shape");
-      check(op.mat + D * D <= mats.size(), "matrix offset");
-      const uint32_t ngroups = 1u << (t - op.nfixed);
-      std::vector<cplx> v(D);
-      for (uint32_t g = 0; g < ngroups; ++g) {
-        uint32_t lb = g;
-        for (uint32_t i = 0; i < op.nfixed; ++i) {
-          uint32_t p = op.fixed[i];
-          lb = ((lb >> p) << (p + 1)) | (lb & ((1u << p) - 1));
+    for (const auto &st : prog.stages) {
+      if (st.kind == qs::dev::kStageRegisters) {
+        check(st.begin <= st.end && st.end <= prog.rops.size(), "register op range");
+        const uint32_t NR = 1u << rb;
+        std::vector<uint32_t> roff(NR, 0);
+        for (uint32_t i = 0; i < NR; ++i)
+          for (uint32_t b = 0; b < rb; ++b)
+            if ((i >> b) & 1) roff[i] |= 1u << st.fixed[b];
+        for (uint32_t b = 1; b < rb; ++b) check(st.fixed[b - 1] < st.fixed[b] && st.fixed[b] < t, "stage bits");
+        std::vector<cplx> v(NR);
+        for (uint32_t g = 0; g < (1u << (t - rb)); ++g) {
+          uint32_t lb = g;
+          for (uint32_t b = 0; b < rb; ++b) {
+            uint32_t p = st.fixed[b];
+            lb = ((lb >> p) << (p + 1)) | (lb & ((1u << p) - 1));
+          }
+          for (uint32_t i = 0; i < NR; ++i) v[i] = lds(lb | roff[i]);
+          for (uint32_t o = st.begin; o < st.end; ++o) {
+            const auto &op = prog.rops[o];
+            if ((base & op.gcmask) != op.gcmask) continue;
+            if ((lb & op.lcmask) != op.lcmask) continue;
+            const uint32_t width = (op.code >> 8) < 4 ? 4 : 16;
+            check(op.mat + width <= prog.mats.size(), "matrix offset");
+            apply_small(v, op.code, op.rcmask, prog.mats.data() + op.mat, rb);
+          }
+          for (uint32_t i = 0; i < NR; ++i) lds(lb | roff[i]) = v[i];
         }
-        lb |= op.lcmask;
-        for (uint32_t j = 0; j < D; ++j) {
-          check(lb + op.off[j] < tsize, "LDS index");
-          v[j] = tile[lb + op.off[j]];
-        }
-        for (uint32_t r = 0; r < D; ++r) {
-          cplx a = 0.0;
-          for (uint32_t c = 0; c < D; ++c) a += mats[op.mat + r * D + c] * v[c];
-          tile[lb + op.off[r]] = a;
+      } else {
+        check(st.begin < prog.tops.size(), "LDS op index");
+        const auto &op = prog.tops[st.begin];
+        if ((base & op.gcmask) != op.gcmask) continue;
+        const uint32_t D = 1u << op.k;
+        check(op.k >= 1 && op.k <= 5 && op.nfixed <= t, "op shape");
+        check(op.mat + D * D <= prog.mats.size(), "matrix offset");
+        const uint32_t ngroups = 1u << (t - op.nfixed);
+        std::vector<cplx> v(D);
+        for (uint32_t g = 0; g < ngroups; ++g) {
+          uint32_t lb = g;
+          for (uint32_t i = 0; i < op.nfixed; ++i) {
+            uint32_t p = op.fixed[i];
+            lb = ((lb >> p) << (p + 1)) | (lb & ((1u << p) - 1));
+          }
+          lb |= op.lcmask;
+          for (uint32_t j = 0; j < D; ++j) v[j] = lds(lb + op.off[j]);
+          for (uint32_t r = 0; r < D; ++r) {
+            cplx a = 0.0;
+            for (uint32_t c = 0; c < D; ++c) a += prog.mats[op.mat + r * D + c] * v[c];
+            lds(lb + op.off[r]) = a;
+          }
         }
       }
     }
-    for (uint32_t l = 0; l < tsize; ++l) psi[base | seg[l >> L] | (l & segmask)] = tile[l];
+    for (uint32_t l = 0; l < tsize; ++l) psi[base | seg[l >> L] | (l & segmask)] = lds(l);
   }
 }
 
@@ -91,7 +173,7 @@ extern "C" {
 
 // gates: k[i] targets at tpos[off..off+k), controls cmask[i], matrix at mats[moff..] (complex
 // interleaved, 4^k entries).  state: 2^n complex interleaved, updated in place.
-int emu_run(unsigned n, unsigned t, unsigned lmin, int n_gates, const uint32_t *k, const uint32_t *tpos,
+int emu_run(unsigned n, unsigned t, unsigned lmin, unsigned rb, int n_gates, const uint32_t *k, const uint32_t *tpos,
             const uint64_t *cmask, const double *mats, double *state, int *n_passes, char *err, int err_len) {
   try {
     std::vector<qs::PGate> gates;
@@ -113,11 +195,9 @@ int emu_run(unsigned n, unsigned t, unsigned lmin, int n_gates, const uint32_t *
       return 0;
     }
     for (auto &ps : passes) {
-      qs::dev::TilePass P;
-      std::vector<qs::dev::TileOp> ops;
-      std::vector<cplx> m;
-      qs::build_tile_pass(gates, ps, std::min(t, n), P, ops, m);
-      replay(reinterpret_cast<cplx *>(state), n, P, ops, m);
+      qs::TileProgram prog;
+      qs::build_tile_pass(gates, ps, std::min(t, n), rb, prog);
+      replay(reinterpret_cast<cplx *>(state), n, prog, rb, rb == 5);
       covered += ps.gates.size();
     }
     check(covered == gates.size(), "every gate scheduled exactly once");

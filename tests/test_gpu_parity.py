@@ -85,6 +85,8 @@ WINDOW_CIRCUITS = [
     ("random21", lambda: circuits.random_layers(21, 8, 10, 33, measure=False)),
     ("qft16", lambda: circuits.allocate(16) + [commands.gate(G.H, [q]) for q in range(16)] + circuits.qft(list(range(16)))),
     ("mixed9", lambda: circuits.mixed_circuit(9, 12, 34)),
+    ("mixed3", lambda: circuits.mixed_circuit(3, 10, 35)),
+    ("mixed12_c", lambda: circuits.mixed_circuit(12, 20, 36)),
 ]
 
 

@@ -34,7 +34,7 @@ def emu():
     return lib
 
 
-def run_emu(lib, n, t, lmin, cmds, state):
+def run_emu(lib, n, t, lmin, cmds, state, rb=5):
     ks, tpos, cm, mats = [], [], [], []
     for c in cmds:
         if c[0] != "gate":
@@ -58,6 +58,7 @@ def run_emu(lib, n, t, lmin, cmds, state):
         ctypes.c_uint(n),
         ctypes.c_uint(t),
         ctypes.c_uint(lmin),
+        ctypes.c_uint(rb),
         ctypes.c_int(len(ks)),
         k.ctypes.data_as(ctypes.c_void_p),
         tp.ctypes.data_as(ctypes.c_void_p),
@@ -92,11 +93,11 @@ CASES = [
 
 
 @pytest.mark.parametrize("name,make", CASES, ids=[c[0] for c in CASES])
-@pytest.mark.parametrize("n,t,lmin", [(8, 8, 5), (12, 10, 4), (14, 10, 5), (15, 12, 4), (16, 13, 5)])
-def test_tile_plan_matches_unfused(emu, name, make, n, t, lmin):
+@pytest.mark.parametrize("n,t,lmin,rb", [(8, 8, 5, 5), (12, 10, 4, 4), (14, 10, 5, 5), (15, 12, 4, 4), (16, 13, 5, 5), (4, 4, 4, 5)])
+def test_tile_plan_matches_unfused(emu, name, make, n, t, lmin, rb):
     cmds = make(n)
     want = oracle_state(cmds, n)
-    got, npass = run_emu(emu, n, t, lmin, cmds, start_state(n))
+    got, npass = run_emu(emu, n, t, lmin, cmds, start_state(n), rb)
     assert np.max(np.abs(got - want)) <= 1e-12
     n_gates = commands.count_gates(cmds)
     assert 1 <= npass <= n_gates
@@ -113,8 +114,9 @@ def test_random_controlled_blocks(emu):
         u, _ = np.linalg.qr(a)
         cmds.append(commands.gate(u, [int(q) for q in picks[:k]], [int(q) for q in picks[k:]]))
     want = oracle_state(cmds, n)
-    got, _ = run_emu(emu, n, 10, 4, cmds, start_state(n))
-    assert np.max(np.abs(got - want)) <= 1e-12
+    for rb in (4, 5):
+        got, _ = run_emu(emu, n, 10, 4, cmds, start_state(n), rb)
+        assert np.max(np.abs(got - want)) <= 1e-12
 
 
 def test_pass_count_random32_layer(emu):
