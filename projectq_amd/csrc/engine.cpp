@@ -6,6 +6,8 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 
@@ -39,6 +41,18 @@ Engine::Engine(uint32_t seed, int precision, int fusion_mode, int planner, unsig
   if (planner < 0 || planner > 2) throw ValueError("unknown planner");
   if (capacity_hint > 36) throw ValueError("capacity hint above 36 qubits");
   QS_HIP(hipGetDevice(&device_));
+  // tile shape: 2^t amplitudes of LDS per workgroup and rb register bits per thread in register
+  // stages.  complex64: t = 12 (32 KiB, four workgroups per CU at 123 VGPRs), rb = 4;
+  // complex128: t = 12 (64 KiB, two workgroups per CU), rb = 4.
+  tile_tmax_ = 12u;
+  tile_rb_ = 4u;
+  if (const char *env = std::getenv("PROJECTQ_AMD_TILE")) {
+    unsigned t = 0, rb = 0;
+    if (std::sscanf(env, "%u:%u", &t, &rb) != 2 || t < 6 || t > (c64_ ? 13u : 12u) || (rb != 4 && rb != 5))
+      throw ValueError("PROJECTQ_AMD_TILE must be \"t:rb\" with 6 <= t <= 13 (12 for c128) and rb 4 or 5");
+    tile_tmax_ = t;
+    tile_rb_ = rb;
+  }
   QS_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
   QS_HIP(hipMalloc(&d_partials_, sizeof(double) * 2 * dev::kReduceGrid));
   QS_HIP(hipMalloc(&d_result_, sizeof(double) * 4));
@@ -367,7 +381,7 @@ void Engine::flush_window() {
 // Tile size: 2^t amplitudes of LDS per workgroup (64 KiB), at most n bits, and small enough
 // that a state of n qubits still spreads over >= 256 tiles when n is large.
 unsigned Engine::tile_bits() const {
-  const unsigned tmax = c64_ ? 13u : 12u;
+  const unsigned tmax = tile_tmax_;
   if (n_ <= 10) return n_;
   return std::min(tmax, std::max(10u, n_ - 8));
 }
@@ -391,7 +405,7 @@ void Engine::flush_window_tiles(std::vector<Gate> &gates) {
 
 void Engine::exec_tile_pass(const std::vector<PGate> &gates, const PPass &ps, unsigned t) {
   TileProgram prog;
-  const unsigned rb = c64_ ? (unsigned)dev::reg_bits<float>() : (unsigned)dev::reg_bits<double>();
+  const unsigned rb = tile_rb_;
   build_tile_pass(gates, ps, t, rb, prog);
   if (prog.stages.empty()) return;
   const uint64_t n_tiles = amps() >> t;
@@ -415,7 +429,7 @@ void Engine::exec_tile_pass(const std::vector<PGate> &gates, const PPass &ps, un
       e1 = take_event();
       QS_HIP(hipEventRecord(e0, stream_));
     }
-    dev::tile_pass<R>((amp_t<R> *)psi_, n_tiles, prog.P, d_st, d_rops, d_tops, dm, stream_);
+    dev::tile_pass<R>((amp_t<R> *)psi_, n_tiles, prog.P, (int)rb, d_st, d_rops, d_tops, dm, stream_);
     if (profiling_) {
       QS_HIP(hipEventRecord(e1, stream_));
       ev_used_.push_back({e0, e1});

@@ -97,6 +97,7 @@ class Engine {
   void flush_window();
   void flush_window_tiles(std::vector<Gate> &gates);
   unsigned tile_bits() const;
+  unsigned tile_tmax_ = 13, tile_rb_ = 5;  // tile shape limits (PROJECTQ_AMD_TILE="t:rb" overrides)
   void exec_tile_pass(const std::vector<PGate> &gates, const PPass &pass, unsigned t);
 
   bool c64_;

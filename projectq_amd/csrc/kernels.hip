@@ -188,14 +188,14 @@ __device__ __forceinline__ void tile_apply(amp_t<R> *__restrict__ tile, uint32_t
 
 // ---- register-stage ops: NR = 2^RB amplitudes of one thread, register index bit i <-> stage
 // bit i; rc = controls among register bits (a compile-time index i passes iff (i & rc) == rc).
-template <typename R, int RB, int B>
+template <typename R, int RB, bool RC, int B>
 __device__ __forceinline__ void rop_general(R *vr, R *vi, uint32_t rc, const amp_t<R> *__restrict__ m) {
   const R ar = m[0].x, ai = m[0].y, br = m[1].x, bi = m[1].y;
   const R cr = m[2].x, ci = m[2].y, dr = m[3].x, di = m[3].y;
 #pragma unroll
   for (int i = 0; i < (1 << RB); ++i) {
     if (i & (1 << B)) continue;
-    if ((i & rc) != rc) continue;
+    if (RC && (i & rc) != rc) continue;
     const int j = i | (1 << B);
     const R xr = vr[i], xi = vi[i], yr = vr[j], yi = vi[j];
     vr[i] = fma(ar, xr, fma(-ai, xi, fma(br, yr, -bi * yi)));
@@ -204,25 +204,40 @@ __device__ __forceinline__ void rop_general(R *vr, R *vi, uint32_t rc, const amp
     vi[j] = fma(cr, xi, fma(ci, xr, fma(dr, yi, di * yr)));
   }
 }
-template <typename R, int RB, int B>
+template <typename R, int RB, bool RC, int B>
+__device__ __forceinline__ void rop_real(R *vr, R *vi, uint32_t rc, const amp_t<R> *__restrict__ m) {
+  const R a = m[0].x, b = m[1].x, c = m[2].x, d = m[3].x;
+#pragma unroll
+  for (int i = 0; i < (1 << RB); ++i) {
+    if (i & (1 << B)) continue;
+    if (RC && (i & rc) != rc) continue;
+    const int j = i | (1 << B);
+    const R xr = vr[i], xi = vi[i], yr = vr[j], yi = vi[j];
+    vr[i] = fma(a, xr, b * yr);
+    vi[i] = fma(a, xi, b * yi);
+    vr[j] = fma(c, xr, d * yr);
+    vi[j] = fma(c, xi, d * yi);
+  }
+}
+template <typename R, int RB, bool RC, int B>
 __device__ __forceinline__ void rop_diag(R *vr, R *vi, uint32_t rc, const amp_t<R> *__restrict__ m) {
   const R ar = m[0].x, ai = m[0].y, dr = m[3].x, di = m[3].y;
 #pragma unroll
   for (int i = 0; i < (1 << RB); ++i) {
-    if ((i & rc) != rc) continue;
+    if (RC && (i & rc) != rc) continue;
     const R fr = (i & (1 << B)) ? dr : ar, fi = (i & (1 << B)) ? di : ai;
     const R xr = vr[i], xi = vi[i];
     vr[i] = fma(fr, xr, -fi * xi);
     vi[i] = fma(fr, xi, fi * xr);
   }
 }
-template <typename R, int RB, int B>
+template <typename R, int RB, bool RC, int B>
 __device__ __forceinline__ void rop_anti(R *vr, R *vi, uint32_t rc, const amp_t<R> *__restrict__ m) {
   const R br = m[1].x, bi = m[1].y, cr = m[2].x, ci = m[2].y;
 #pragma unroll
   for (int i = 0; i < (1 << RB); ++i) {
     if (i & (1 << B)) continue;
-    if ((i & rc) != rc) continue;
+    if (RC && (i & rc) != rc) continue;
     const int j = i | (1 << B);
     const R xr = vr[i], xi = vi[i], yr = vr[j], yi = vi[j];
     vr[i] = fma(br, yr, -bi * yi);
@@ -231,12 +246,12 @@ __device__ __forceinline__ void rop_anti(R *vr, R *vi, uint32_t rc, const amp_t<
     vi[j] = fma(cr, xi, ci * xr);
   }
 }
-template <typename R, int RB, int B>
+template <typename R, int RB, bool RC, int B>
 __device__ __forceinline__ void rop_x(R *vr, R *vi, uint32_t rc, const amp_t<R> *__restrict__) {
 #pragma unroll
   for (int i = 0; i < (1 << RB); ++i) {
     if (i & (1 << B)) continue;
-    if ((i & rc) != rc) continue;
+    if (RC && (i & rc) != rc) continue;
     const int j = i | (1 << B);
     const R xr = vr[i], xi = vi[i];
     vr[i] = vr[j];
@@ -245,12 +260,12 @@ __device__ __forceinline__ void rop_x(R *vr, R *vi, uint32_t rc, const amp_t<R> 
     vi[j] = xi;
   }
 }
-template <typename R, int RB, int B0, int B1>
+template <typename R, int RB, bool RC, int B0, int B1>
 __device__ __forceinline__ void rop_general2(R *vr, R *vi, uint32_t rc, const amp_t<R> *__restrict__ m) {
 #pragma unroll
   for (int i = 0; i < (1 << RB); ++i) {
     if (i & ((1 << B0) | (1 << B1))) continue;
-    if ((i & rc) != rc) continue;
+    if (RC && (i & rc) != rc) continue;
     const int idx[4] = {i, i | (1 << B0), i | (1 << B1), i | (1 << B0) | (1 << B1)};
     R xr[4], xi[4];
 #pragma unroll
@@ -272,11 +287,11 @@ __device__ __forceinline__ void rop_general2(R *vr, R *vi, uint32_t rc, const am
     }
   }
 }
-template <typename R, int RB, int B0, int B1>
+template <typename R, int RB, bool RC, int B0, int B1>
 __device__ __forceinline__ void rop_diag2(R *vr, R *vi, uint32_t rc, const amp_t<R> *__restrict__ m) {
 #pragma unroll
   for (int i = 0; i < (1 << RB); ++i) {
-    if ((i & rc) != rc) continue;
+    if (RC && (i & rc) != rc) continue;
     const int s = ((i >> B0) & 1) | (((i >> B1) & 1) << 1);
     const amp_t<R> f = m[s * 5];  // diagonal element s of the 4x4 matrix
     const R xr = vr[i], xi = vi[i];
@@ -287,34 +302,35 @@ __device__ __forceinline__ void rop_diag2(R *vr, R *vi, uint32_t rc, const amp_t
 
 #define QS_ROP1(FN)                                                                     \
   switch (b0) {                                                                         \
-    case 0: FN<R, RB, 0>(vr, vi, rc, m); break;                                         \
-    case 1: FN<R, RB, 1>(vr, vi, rc, m); break;                                         \
-    case 2: FN<R, RB, 2>(vr, vi, rc, m); break;                                         \
-    case 3: FN<R, RB, 3>(vr, vi, rc, m); break;                                         \
+    case 0: FN<R, RB, RC, 0>(vr, vi, rc, m); break;                                         \
+    case 1: FN<R, RB, RC, 1>(vr, vi, rc, m); break;                                         \
+    case 2: FN<R, RB, RC, 2>(vr, vi, rc, m); break;                                         \
+    case 3: FN<R, RB, RC, 3>(vr, vi, rc, m); break;                                         \
     default:                                                                            \
-      if constexpr (RB > 4) FN<R, RB, 4>(vr, vi, rc, m);                                \
+      if constexpr (RB > 4) FN<R, RB, RC, 4>(vr, vi, rc, m);                                \
       break;                                                                            \
   }
 #define QS_ROP2(FN)                                                                     \
   switch (b0 * 8 + b1) {                                                                \
-    case 0 * 8 + 1: FN<R, RB, 0, 1>(vr, vi, rc, m); break;                              \
-    case 0 * 8 + 2: FN<R, RB, 0, 2>(vr, vi, rc, m); break;                              \
-    case 0 * 8 + 3: FN<R, RB, 0, 3>(vr, vi, rc, m); break;                              \
-    case 1 * 8 + 2: FN<R, RB, 1, 2>(vr, vi, rc, m); break;                              \
-    case 1 * 8 + 3: FN<R, RB, 1, 3>(vr, vi, rc, m); break;                              \
-    case 2 * 8 + 3: FN<R, RB, 2, 3>(vr, vi, rc, m); break;                              \
-    case 0 * 8 + 4: if constexpr (RB > 4) FN<R, RB, 0, 4>(vr, vi, rc, m); break;        \
-    case 1 * 8 + 4: if constexpr (RB > 4) FN<R, RB, 1, 4>(vr, vi, rc, m); break;        \
-    case 2 * 8 + 4: if constexpr (RB > 4) FN<R, RB, 2, 4>(vr, vi, rc, m); break;        \
-    default: if constexpr (RB > 4) FN<R, RB, 3, 4>(vr, vi, rc, m); break;               \
+    case 0 * 8 + 1: FN<R, RB, RC, 0, 1>(vr, vi, rc, m); break;                              \
+    case 0 * 8 + 2: FN<R, RB, RC, 0, 2>(vr, vi, rc, m); break;                              \
+    case 0 * 8 + 3: FN<R, RB, RC, 0, 3>(vr, vi, rc, m); break;                              \
+    case 1 * 8 + 2: FN<R, RB, RC, 1, 2>(vr, vi, rc, m); break;                              \
+    case 1 * 8 + 3: FN<R, RB, RC, 1, 3>(vr, vi, rc, m); break;                              \
+    case 2 * 8 + 3: FN<R, RB, RC, 2, 3>(vr, vi, rc, m); break;                              \
+    case 0 * 8 + 4: if constexpr (RB > 4) FN<R, RB, RC, 0, 4>(vr, vi, rc, m); break;        \
+    case 1 * 8 + 4: if constexpr (RB > 4) FN<R, RB, RC, 1, 4>(vr, vi, rc, m); break;        \
+    case 2 * 8 + 4: if constexpr (RB > 4) FN<R, RB, RC, 2, 4>(vr, vi, rc, m); break;        \
+    default: if constexpr (RB > 4) FN<R, RB, RC, 3, 4>(vr, vi, rc, m); break;               \
   }
 
-template <typename R, int RB>
+template <typename R, int RB, bool RC>
 __device__ __forceinline__ void reg_op(uint32_t code, R *vr, R *vi, uint32_t rc,
                                        const amp_t<R> *__restrict__ m) {
   const uint32_t kind = code >> 8, b0 = code & 15u, b1 = (code >> 4) & 15u;
   switch (kind) {
     case kOp1General: QS_ROP1(rop_general) break;
+    case kOp1Real: QS_ROP1(rop_real) break;
     case kOp1Diag: QS_ROP1(rop_diag) break;
     case kOp1Anti: QS_ROP1(rop_anti) break;
     case kOp1X: QS_ROP1(rop_x) break;
@@ -360,7 +376,9 @@ __device__ __forceinline__ void reg_stage(amp_t<R> *__restrict__ tile, uint32_t 
       if ((base & gc) != gc) continue;
       const uint32_t lc = op->lcmask;
       if ((lb & lc) != lc) continue;
-      reg_op<R, RB>(op->code, vr, vi, op->rcmask, mats + op->mat);
+      const uint32_t rc = op->rcmask;
+      if (rc) reg_op<R, RB, true>(op->code, vr, vi, rc, mats + op->mat);
+      else reg_op<R, RB, false>(op->code, vr, vi, 0u, mats + op->mat);
     }
 #pragma unroll
     for (int i = 0; i < NR; ++i) {
@@ -882,10 +900,9 @@ void apply_block(amp_t<R> *psi, uint64_t n_groups, const BitInsert &ins, uint64_
   }
 }
 
-template <typename R>
-void tile_pass(amp_t<R> *psi, uint64_t n_tiles, const TilePass &P, const TileStage *d_stages,
-               const RegOp *d_rops, const TileOp *d_tops, const amp_t<R> *d_mats, hipStream_t s) {
-  constexpr int RB = reg_bits<R>();
+template <typename R, int RB>
+void launch_tile_pass(amp_t<R> *psi, uint64_t n_tiles, const TilePass &P, const TileStage *d_stages,
+                      const RegOp *d_rops, const TileOp *d_tops, const amp_t<R> *d_mats, hipStream_t s) {
   static bool attr_set = false;
   const size_t lds = sizeof(amp_t<R>) << P.t;
   if (!attr_set) {
@@ -893,12 +910,20 @@ void tile_pass(amp_t<R> *psi, uint64_t n_tiles, const TilePass &P, const TileSta
                                (int)(sizeof(amp_t<R>) << kTileMaxBits)));
     attr_set = true;
   }
-  if (P.h > (uint32_t)kTileMaxHigh || P.L < 1 || P.t > (uint32_t)kTileMaxBits)
-    throw ValueError("tile pass: bad tile shape");
   const uint64_t grid = std::min<uint64_t>(n_tiles, 1u << 20);
   hipLaunchKernelGGL((k_tile_pass<R, RB>), dim3((unsigned)grid), dim3(kBlock), lds, s, psi, n_tiles, P,
                      d_stages, d_rops, d_tops, d_mats);
   QS_HIP(hipGetLastError());
+}
+
+template <typename R>
+void tile_pass(amp_t<R> *psi, uint64_t n_tiles, const TilePass &P, int rb, const TileStage *d_stages,
+               const RegOp *d_rops, const TileOp *d_tops, const amp_t<R> *d_mats, hipStream_t s) {
+  if (P.h > (uint32_t)kTileMaxHigh || P.L < 1 || P.t > (uint32_t)kTileMaxBits)
+    throw ValueError("tile pass: bad tile shape");
+  if (rb == 5) launch_tile_pass<R, 5>(psi, n_tiles, P, d_stages, d_rops, d_tops, d_mats, s);
+  else if (rb == 4) launch_tile_pass<R, 4>(psi, n_tiles, P, d_stages, d_rops, d_tops, d_mats, s);
+  else throw ValueError("tile pass: register bits must be 4 or 5");
 }
 
 template <typename R>
@@ -1036,7 +1061,7 @@ void math_accumulate(const amp_t<R> *src, amp_t<R> *dst, uint64_t n, const MathA
 
 // ------------------------------------------------------------------------------ instantiations
 #define QS_INSTANTIATE(R)                                                                          \
-  template void tile_pass<R>(amp_t<R> *, uint64_t, const TilePass &, const TileStage *,           \
+  template void tile_pass<R>(amp_t<R> *, uint64_t, const TilePass &, int, const TileStage *,      \
                              const RegOp *, const TileOp *, const amp_t<R> *, hipStream_t);       \
   template void apply_block<R>(amp_t<R> *, uint64_t, const BitInsert &, uint64_t,                 \
                                const uint32_t *, int, const amp_t<R> *, hipStream_t);             \

@@ -27,13 +27,9 @@ struct MathArgs {
   uint8_t pos[kMaxRegs][32];
 };
 
-// tile passes (TilePass / TileStage / RegOp / TileOp in layout.hpp)
+// tile passes (TilePass / TileStage / RegOp / TileOp in layout.hpp); rb = register bits (4, 5)
 template <typename R>
-constexpr int reg_bits() {
-  return sizeof(R) == 4 ? 5 : 4;
-}
-template <typename R>
-void tile_pass(amp_t<R> *psi, uint64_t n_tiles, const TilePass &P, const TileStage *d_stages,
+void tile_pass(amp_t<R> *psi, uint64_t n_tiles, const TilePass &P, int rb, const TileStage *d_stages,
                const RegOp *d_rops, const TileOp *d_tops, const amp_t<R> *d_mats, hipStream_t s);
 
 // dense gate block: psi[base + off(r)] = sum_c M[r][c] psi[base + off(c)] for every base with

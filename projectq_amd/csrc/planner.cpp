@@ -159,7 +159,9 @@ uint32_t classify1(const std::vector<cplx> &m) {
   const cplx zero(0.0, 0.0), one(1.0, 0.0);
   if (m[0] == zero && m[3] == zero) return (m[1] == one && m[2] == one) ? dev::kOp1X : dev::kOp1Anti;
   if (m[1] == zero && m[2] == zero) return dev::kOp1Diag;
-  return dev::kOp1General;
+  bool real = true;
+  for (int i = 0; i < 4; ++i) real = real && m[i].imag() == 0.0;
+  return real ? dev::kOp1Real : dev::kOp1General;
 }
 
 bool is_diag4(const std::vector<cplx> &m) {

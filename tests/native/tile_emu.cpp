@@ -37,8 +37,8 @@ uint32_t slot(uint32_t x, bool c64) { return c64 ? qs::dev::lds_slot_c64(x) : qs
 void apply_small(std::vector<cplx> &v, uint32_t code, uint32_t rc, const cplx *m, unsigned rb) {
   const uint32_t kind = code >> 8, b0 = code & 15u, b1 = (code >> 4) & 15u;
   const uint32_t NR = 1u << rb;
-  check(b0 < rb && (kind < 4 || (b1 < rb && b0 < b1)), "register op bits");
-  if (kind < 4) {
+  check(b0 < rb && (kind < 4 || kind == qs::dev::kOp1Real || (b1 < rb && b0 < b1)), "register op bits");
+  if (kind < 4 || kind == qs::dev::kOp1Real) {
     for (uint32_t i = 0; i < NR; ++i) {
       if ((i & rc) != rc) continue;
       if (kind == qs::dev::kOp1Diag) {
@@ -48,7 +48,7 @@ void apply_small(std::vector<cplx> &v, uint32_t code, uint32_t rc, const cplx *m
       if ((i >> b0) & 1) continue;
       const uint32_t j = i | (1u << b0);
       const cplx x = v[i], y = v[j];
-      if (kind == qs::dev::kOp1General) {
+      if (kind == qs::dev::kOp1General || kind == qs::dev::kOp1Real) {
         v[i] = m[0] * x + m[1] * y;
         v[j] = m[2] * x + m[3] * y;
       } else if (kind == qs::dev::kOp1Anti) {
@@ -132,7 +132,8 @@ void replay(cplx *psi, unsigned n, const qs::TileProgram &prog, unsigned rb, boo
             const auto &op = prog.rops[o];
             if ((base & op.gcmask) != op.gcmask) continue;
             if ((lb & op.lcmask) != op.lcmask) continue;
-            const uint32_t width = (op.code >> 8) < 4 ? 4 : 16;
+            const uint32_t kd = op.code >> 8;
+            const uint32_t width = (kd < 4 || kd == qs::dev::kOp1Real) ? 4 : 16;
             check(op.mat + width <= prog.mats.size(), "matrix offset");
             apply_small(v, op.code, op.rcmask, prog.mats.data() + op.mat, rb);
           }
