@@ -163,6 +163,30 @@ int qs_reset_stats(qs_simulator *sim);
  * (precisions may differ); accumulated in FP64.  result = (re, im). */
 int qs_overlap(qs_simulator *a, qs_simulator *b, double *result);
 
+/* ---- batched circuits (extension, no counterpart in the reference) ---------------------------
+ * B independent circuits of uneven qubit counts, each on its own register starting in |0..0>,
+ * run together on one GPU and reduced to <psi_b|H_b|psi_b>.  Used by variational workloads.
+ * Circuit b owns gates [gate_offsets[b], gate_offsets[b+1]); gate g has gate_k[g] targets then
+ * gate_nctrl[g] controls, listed in gate_qubits (qubit indices 0..n_b-1), and a 2^k x 2^k matrix
+ * (interleaved complex, concatenated).  Circuit b's operator H_b owns terms
+ * [term_offsets[b], term_offsets[b+1]); term t has ops [op_offsets[t], op_offsets[t+1]) of
+ * (op_qubits, op_kinds 'X'|'Y'|'Z') and a real coefficient.  autotune != 0 times candidate tile
+ * shapes on the batch and keeps the fastest. */
+typedef struct qs_batch qs_batch;
+int qs_batch_create(int precision, size_t n_circuits, const uint32_t *n_qubits,
+                    const size_t *gate_offsets, const uint32_t *gate_k, const uint32_t *gate_nctrl,
+                    const uint32_t *gate_qubits, const double *matrices,
+                    const size_t *term_offsets, const size_t *op_offsets,
+                    const uint32_t *op_qubits, const uint8_t *op_kinds, const double *coeffs,
+                    int autotune, qs_batch **out);
+/* One run: a single hipGraph launch (use_graph != 0) or the same kernels launched one by one.
+ * energies[b] receives <psi_b|H_b|psi_b>; *gpu_ms the GPU time of the run. */
+int qs_batch_run(qs_batch *batch, int use_graph, double *energies, double *gpu_ms);
+/* Tile shape in use (after autotuning) and the number of pass launches per run. */
+int qs_batch_shape(qs_batch *batch, uint32_t *tile_bits, uint32_t *reg_bits, uint32_t *low_bits,
+                   uint32_t *launches);
+void qs_batch_destroy(qs_batch *batch);
+
 #ifdef __cplusplus
 }
 #endif

@@ -4,7 +4,7 @@ import ctypes
 import os
 import subprocess
 
-from projectq_amd._binding import CAbiSimulator, bind_library
+from projectq_amd._binding import CAbiBatch, CAbiSimulator, bind_library
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "liboracle_sim.so")
@@ -32,3 +32,10 @@ class OracleSimulator(CAbiSimulator):
 
     def __init__(self, seed):
         super().__init__(library(), seed, precision="c128", fusion="reference", planner="greedy")
+
+
+class OracleBatch(CAbiBatch):
+    """Batched circuits on the CPU oracle: each circuit unfused, then its expectation value."""
+
+    def __init__(self, circuits, operators):
+        super().__init__(library(), circuits, operators, precision="c128", autotune=False)

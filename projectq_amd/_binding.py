@@ -103,6 +103,32 @@ _SIGNATURES = {
     ),
     "qs_reset_stats": (ctypes.c_int, [ctypes.c_void_p]),
     "qs_overlap": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, _f64p]),
+    "qs_batch_create": (
+        ctypes.c_int,
+        [
+            ctypes.c_int,
+            ctypes.c_size_t,
+            _u32p,
+            _szp,
+            _u32p,
+            _u32p,
+            _u32p,
+            _f64p,
+            _szp,
+            _szp,
+            _u32p,
+            _u8p,
+            _f64p,
+            ctypes.c_int,
+            ctypes.POINTER(ctypes.c_void_p),
+        ],
+    ),
+    "qs_batch_run": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _f64p, _f64p]),
+    "qs_batch_shape": (
+        ctypes.c_int,
+        [ctypes.c_void_p, _u32p, _u32p, _u32p, _u32p],
+    ),
+    "qs_batch_destroy": (None, [ctypes.c_void_p]),
 }
 
 #: every symbol the C ABI declares (include/projectq_amd.h)
@@ -479,6 +505,102 @@ class CAbiSimulator:
         """simulator.hpp:529-532: (dict id -> bit position, list of complex amplitudes)."""
         mapping, state = self.cheat_numpy()
         return mapping, state.tolist()
+
+
+class CAbiBatch:
+    """Batched circuits over the C ABI (qs_batch_*): `circuits` is a list of (n_qubits, gate
+    commands) with qubits 0..n-1 starting in |0...0>; `operators` one Pauli operator per circuit
+    in the backend's term format [(list[(qubit, 'X'|'Y'|'Z')], real coefficient)]."""
+
+    def __init__(self, lib, circuits, operators, precision="c64", autotune=True):
+        if len(circuits) != len(operators):
+            raise ValueError("one operator per circuit")
+        self._lib = lib
+        self._handle = None
+        nq, goff, gk, gnc, gq, mats = [], [0], [], [], [], []
+        toff, ooff, oq, ok, co = [0], [0], [], [], []
+        for (n, cmds), terms in zip(circuits, operators):
+            nq.append(int(n))
+            for c in cmds:
+                if c[0] != "gate":
+                    continue
+                _, m, targets, controls = c
+                gk.append(len(targets))
+                gnc.append(len(controls))
+                gq.extend(int(q) for q in targets)
+                gq.extend(int(q) for q in controls)
+                mats.append(np.asarray(m, dtype=np.complex128).reshape(-1))
+            goff.append(len(gk))
+            for term, coeff in terms:
+                for q, op in term:
+                    oq.append(int(q))
+                    ok.append(ord(str(op)))
+                ooff.append(len(oq))
+                co.append(float(coeff))
+            toff.append(len(co))
+        arrs = dict(
+            nq=_u32(nq),
+            goff=np.asarray(goff, np.uintp),
+            gk=_u32(gk) if gk else np.zeros(1, np.uint32),
+            gnc=_u32(gnc) if gnc else np.zeros(1, np.uint32),
+            gq=_u32(gq) if gq else np.zeros(1, np.uint32),
+            mats=np.ascontiguousarray(np.concatenate(mats)).view(np.float64) if mats else np.zeros(2),
+            toff=np.asarray(toff, np.uintp),
+            ooff=np.asarray(ooff, np.uintp),
+            oq=_u32(oq) if oq else np.zeros(1, np.uint32),
+            ok=np.asarray(ok if ok else [0], np.uint8),
+            co=np.asarray(co if co else [0.0], np.float64),
+        )
+        self._keep = arrs
+        self.n_circuits = len(nq)
+        handle = ctypes.c_void_p()
+        status = lib.qs_batch_create(
+            PRECISIONS[precision],
+            len(nq),
+            _ptr(arrs["nq"], ctypes.c_uint32),
+            _ptr(arrs["goff"], ctypes.c_size_t),
+            _ptr(arrs["gk"], ctypes.c_uint32),
+            _ptr(arrs["gnc"], ctypes.c_uint32),
+            _ptr(arrs["gq"], ctypes.c_uint32),
+            _ptr(arrs["mats"], ctypes.c_double),
+            _ptr(arrs["toff"], ctypes.c_size_t),
+            _ptr(arrs["ooff"], ctypes.c_size_t),
+            _ptr(arrs["oq"], ctypes.c_uint32),
+            _ptr(arrs["ok"], ctypes.c_uint8),
+            _ptr(arrs["co"], ctypes.c_double),
+            1 if autotune else 0,
+            ctypes.byref(handle),
+        )
+        self._check(status)
+        self._handle = handle
+
+    def _check(self, status):
+        if status == QS_OK:
+            return
+        msg = (self._lib.qs_last_error() or b"").decode(errors="replace")
+        if status == QS_ERR_VALUE:
+            raise ValueError(msg)
+        if status == QS_ERR_MEMORY:
+            raise MemoryError(msg)
+        raise RuntimeError(msg)
+
+    def run(self, graph=True):
+        """One run of every circuit: (energies ndarray, GPU milliseconds)."""
+        out = np.zeros(self.n_circuits, dtype=np.float64)
+        ms = ctypes.c_double()
+        self._check(self._lib.qs_batch_run(self._handle, 1 if graph else 0, _ptr(out, ctypes.c_double), ctypes.byref(ms)))
+        return out, ms.value
+
+    def shape(self):
+        t, rb, low, launches = (ctypes.c_uint32() for _ in range(4))
+        self._check(self._lib.qs_batch_shape(self._handle, ctypes.byref(t), ctypes.byref(rb), ctypes.byref(low), ctypes.byref(launches)))
+        return {"tile_bits": t.value, "reg_bits": rb.value, "low_bits": low.value, "launches": launches.value}
+
+    def __del__(self):
+        handle = getattr(self, "_handle", None)
+        if handle is not None and handle.value:
+            self._lib.qs_batch_destroy(handle)
+            self._handle = None
 
 
 def qubit_count_for(n_amps):

@@ -8,6 +8,7 @@
 #include <cstring>
 #include <string>
 
+#include "batch.hpp"
 #include "engine.hpp"
 
 struct qs_simulator {
@@ -224,4 +225,81 @@ API int qs_overlap(qs_simulator *a, qs_simulator *b, double *result) {
     result[0] = v.real();
     result[1] = v.imag();
   });
+}
+
+struct qs_batch {
+  qs::Batch batch;
+  int device;
+  qs_batch(bool c64, std::vector<qs::BatchCircuit> c, bool tune, int dev) : batch(c64, std::move(c), tune), device(dev) {}
+};
+
+API int qs_batch_create(int precision, size_t n_circuits, const uint32_t *n_qubits, const size_t *gate_offsets,
+                        const uint32_t *gate_k, const uint32_t *gate_nctrl, const uint32_t *gate_qubits,
+                        const double *matrices, const size_t *term_offsets, const size_t *op_offsets,
+                        const uint32_t *op_qubits, const uint8_t *op_kinds, const double *coeffs, int autotune,
+                        qs_batch **out) {
+  return guarded(nullptr, [&] {
+    if (precision != QS_C128 && precision != QS_C64) throw qs::ValueError("batch: unknown precision");
+    std::vector<qs::BatchCircuit> circ(n_circuits);
+    size_t qi = 0, mi = 0;
+    for (size_t b = 0; b < n_circuits; ++b) {
+      circ[b].n = n_qubits[b];
+      if (circ[b].n < 1 || circ[b].n > 30) throw qs::ValueError("batch: circuits hold 1..30 qubits");
+      for (size_t g = gate_offsets[b]; g < gate_offsets[b + 1]; ++g) {
+        qs::PGate pg;
+        const uint32_t k = gate_k[g];
+        if (k < 1 || k > 5) throw qs::ValueError("batch: gates act on 1..5 qubits");
+        for (uint32_t l = 0; l < k; ++l) pg.tpos.push_back(gate_qubits[qi++]);
+        for (uint32_t l = 0; l < gate_nctrl[g]; ++l) {
+          const uint32_t c = gate_qubits[qi++];
+          if (c >= 64) throw qs::ValueError("batch: control outside its circuit");
+          pg.cmask |= uint64_t(1) << c;
+        }
+        const size_t d = size_t(1) << k;
+        pg.m.resize(d * d);
+        for (size_t e = 0; e < d * d; ++e, ++mi) pg.m[e] = qs::cplx(matrices[2 * mi], matrices[2 * mi + 1]);
+        circ[b].gates.push_back(std::move(pg));
+      }
+      for (size_t t = term_offsets[b]; t < term_offsets[b + 1]; ++t) {
+        std::vector<std::pair<uint32_t, char>> ops;
+        for (size_t o = op_offsets[t]; o < op_offsets[t + 1]; ++o) {
+          if (op_qubits[o] >= circ[b].n) throw qs::ValueError("batch: Pauli string outside its circuit");
+          ops.push_back({op_qubits[o], (char)op_kinds[o]});
+        }
+        qs::dev::PauliTerm pt{};
+        qs::cplx w;
+        qs::encode_pauli(ops, &pt.x, &pt.z, &w);
+        const qs::cplx c = coeffs[t] * w;
+        pt.c_re = c.real();
+        pt.c_im = c.imag();
+        circ[b].terms.push_back(pt);
+      }
+    }
+    int dev = 0;
+    QS_HIP(hipGetDevice(&dev));
+    *out = new qs_batch(precision == QS_C64, std::move(circ), autotune != 0, dev);
+  });
+}
+
+API int qs_batch_run(qs_batch *b, int use_graph, double *energies, double *gpu_ms) {
+  return guarded(nullptr, [&] {
+    QS_HIP(hipSetDevice(b->device));
+    double ms = b->batch.run(use_graph != 0, energies);
+    if (gpu_ms) *gpu_ms = ms;
+  });
+}
+
+API int qs_batch_shape(qs_batch *b, uint32_t *t, uint32_t *rb, uint32_t *lmin, uint32_t *launches) {
+  return guarded(nullptr, [&] {
+    *t = b->batch.shape().tmax;
+    *rb = b->batch.shape().rb;
+    *lmin = b->batch.shape().lmin;
+    *launches = b->batch.launches();
+  });
+}
+
+API void qs_batch_destroy(qs_batch *b) {
+  if (!b) return;
+  (void)hipSetDevice(b->device);
+  delete b;
 }

@@ -637,46 +637,53 @@ void Engine::emulate_math_callback(const std::function<bool(int32_t *, size_t)> 
 // x ^= q, z ^= q.  (P psi)_i = omega (-1)^{popcount((i^x)&z)} psi_{i^x}.
 void Engine::pauli_encode(const PauliString &t, const std::vector<unsigned> &ids, uint64_t *x,
                           uint64_t *z, cplx *omega) const {
-  uint64_t X = 0, Z = 0;
-  cplx w(1.0, 0.0);
+  std::vector<std::pair<uint32_t, char>> ops;
   for (auto &op : t.ops) {
     if (op.first >= ids.size()) throw ValueError("qubit_operator acts on more qubits than contained in the qureg.");
-    const uint64_t q = uint64_t(1) << pos_.at(ids[op.first]);
-    const double sgn = (X & q) ? -1.0 : 1.0;
-    switch (op.second) {
-      case 'X': X ^= q; break;
-      case 'Z': w *= sgn; Z ^= q; break;
-      case 'Y': w *= cplx(0.0, sgn); X ^= q; Z ^= q; break;
-      default: throw ValueError("Pauli operator must be X, Y or Z");
-    }
+    if (op.second != 'X' && op.second != 'Y' && op.second != 'Z') throw ValueError("Pauli operator must be X, Y or Z");
+    ops.push_back({pos_.at(ids[op.first]), op.second});
   }
-  *x = X;
-  *z = Z;
-  *omega = w;
+  encode_pauli(ops, x, z, omega);
 }
 
-// simulator.hpp:292-322: sum_t c_t Re<psi|P_t|psi>, one read-only pass per string.
+// simulator.hpp:292-322: sum_t c_t Re<psi|P_t|psi>.  A state that fits in the last-level cache
+// is read once for all strings (one FP64 accumulator per thread: the coefficients and phases
+// are folded into each string); a larger state is read once per string so every pass streams.
 double Engine::get_expectation_value(const std::vector<PauliString> &terms,
                                      const std::vector<double> &coeffs,
                                      const std::vector<unsigned> &ids) {
   run();
   check_known(ids, "get_expectation_value");
+  std::vector<dev::PauliTerm> pt(terms.size());
+  std::vector<cplx> omega(terms.size());
+  for (size_t t = 0; t < terms.size(); ++t) pauli_encode(terms[t], ids, &pt[t].x, &pt[t].z, &omega[t]);
+  const bool cached = amps() * amp_bytes() <= (uint64_t(64) << 20);
+  if (cached && !pt.empty()) {
+    for (size_t t = 0; t < pt.size(); ++t) {
+      const cplx c = coeffs[t] * omega[t];
+      pt[t].c_re = c.real();
+      pt[t].c_im = c.imag();
+    }
+    const dev::PauliTerm *d_terms = (const dev::PauliTerm *)upload(pt.data(), pt.size() * sizeof(dev::PauliTerm));
+    dispatch([&](auto tag) {
+      using R = decltype(tag);
+      dev::pauli_energy<R>((const amp_t<R> *)psi_, amps(), d_terms, (int)pt.size(), d_partials_, stream_);
+    });
+    return finish(dev::kReduceGrid, 1);
+  }
   double total = 0.0;
   for (size_t t = 0; t < terms.size(); ++t) {
-    uint64_t x, z;
-    cplx w;
-    pauli_encode(terms[t], ids, &x, &z, &w);
     double delta;
-    if (x == 0 && z == 0) {
-      delta = w.real() * masked_norm(0, 0);
+    if (pt[t].x == 0 && pt[t].z == 0) {
+      delta = omega[t].real() * masked_norm(0, 0);
     } else {
       dispatch([&](auto tag) {
         using R = decltype(tag);
-        dev::pauli_expect<R>((const amp_t<R> *)psi_, amps(), x, z, d_partials_, stream_);
+        dev::pauli_expect<R>((const amp_t<R> *)psi_, amps(), pt[t].x, pt[t].z, d_partials_, stream_);
       });
       double si;
       double sr = finish(dev::kReduceGrid, 2, &si);
-      delta = (w * cplx(sr, si)).real();
+      delta = (omega[t] * cplx(sr, si)).real();
     }
     total += coeffs[t] * delta;
   }

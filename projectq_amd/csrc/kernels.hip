@@ -10,6 +10,7 @@
 #include <hipcub/hipcub.hpp>
 
 #include <algorithm>
+#include <cstdlib>
 #include <type_traits>
 
 #include "kernels.hpp"
@@ -402,6 +403,61 @@ struct Vec16<double> {
   static constexpr int n = 1;
 };
 
+// One tile of one pass: load the 2^t amplitudes into LDS, run the stages, store them back.
+// `seg` holds the 2^h segment offsets of the pass (shared memory, filled by the caller).
+template <typename R, int RB>
+__device__ __forceinline__ void tile_body(amp_t<R> *__restrict__ psi, uint64_t tix, const TilePass &P,
+                                          const uint64_t *seg, amp_t<R> *__restrict__ tile,
+                                          const TileStage *__restrict__ stages,
+                                          const RegOp *__restrict__ rops, const TileOp *__restrict__ tops,
+                                          const amp_t<R> *__restrict__ mats) {
+  using V = typename Vec16<R>::type;
+  constexpr int VN = Vec16<R>::n;
+  const uint32_t t = P.t, L = P.L;
+  const uint32_t tsize = 1u << t, segmask = (1u << L) - 1;
+  const uint64_t base = insert_zeros(tix, P.tiles);
+  for (uint32_t l = threadIdx.x * VN; l < tsize; l += kBlock * VN) {
+    const V *src = reinterpret_cast<const V *>(psi + (base | seg[l >> L] | (l & segmask)));
+    *reinterpret_cast<V *>(tile + slot<R>(l)) = *src;
+  }
+  __syncthreads();
+  for (uint32_t s = 0; s < P.n_stages; ++s) {
+    const TileStage *st = stages + s;
+    if (st->kind == kStageRegisters) {
+      reg_stage<R, RB>(tile, t, base, st, rops, mats);
+    } else {
+      const TileOp *op = tops + st->begin;
+      const uint64_t gc = op->gcmask;
+      if ((base & gc) == gc) {
+        const amp_t<R> *m = mats + op->mat;
+        switch (op->k) {
+          case 1: tile_apply<R, 1>(tile, t, op, m); break;
+          case 2: tile_apply<R, 2>(tile, t, op, m); break;
+          case 3: tile_apply<R, 3>(tile, t, op, m); break;
+          case 4: tile_apply<R, 4>(tile, t, op, m); break;
+          default: tile_apply<R, 5>(tile, t, op, m); break;
+        }
+      }
+    }
+    __syncthreads();
+  }
+  for (uint32_t l = threadIdx.x * VN; l < tsize; l += kBlock * VN) {
+    V *dst = reinterpret_cast<V *>(psi + (base | seg[l >> L] | (l & segmask)));
+    *dst = *reinterpret_cast<const V *>(tile + slot<R>(l));
+  }
+  __syncthreads();
+}
+
+__device__ __forceinline__ void fill_segments(const TilePass &P, uint64_t *seg) {
+  for (uint32_t sgi = threadIdx.x; sgi < (1u << P.h); sgi += kBlock) {
+    uint64_t o = 0;
+    for (uint32_t b = 0; b < P.h; ++b)
+      if ((sgi >> b) & 1) o |= uint64_t(1) << P.hpos[b];
+    seg[sgi] = o;
+  }
+  __syncthreads();
+}
+
 template <typename R, int RB>
 __global__ __launch_bounds__(kBlock) void k_tile_pass(amp_t<R> *__restrict__ psi, uint64_t n_tiles,
                                                       TilePass P, const TileStage *__restrict__ stages,
@@ -410,51 +466,88 @@ __global__ __launch_bounds__(kBlock) void k_tile_pass(amp_t<R> *__restrict__ psi
                                                       const amp_t<R> *__restrict__ mats) {
   extern __shared__ __align__(16) unsigned char smem[];
   __shared__ uint64_t seg[1 << kTileMaxHigh];
-  using V = typename Vec16<R>::type;
-  constexpr int VN = Vec16<R>::n;
   amp_t<R> *tile = reinterpret_cast<amp_t<R> *>(smem);
-  const uint32_t t = P.t, L = P.L, h = P.h;
-  const uint32_t tsize = 1u << t, segmask = (1u << L) - 1;
-  for (uint32_t sgi = threadIdx.x; sgi < (1u << h); sgi += kBlock) {
-    uint64_t o = 0;
-    for (uint32_t b = 0; b < h; ++b)
-      if ((sgi >> b) & 1) o |= uint64_t(1) << P.hpos[b];
-    seg[sgi] = o;
-  }
-  __syncthreads();
-  for (uint64_t tix = blockIdx.x; tix < n_tiles; tix += gridDim.x) {
-    const uint64_t base = insert_zeros(tix, P.tiles);
-    for (uint32_t l = threadIdx.x * VN; l < tsize; l += kBlock * VN) {
-      const V *src = reinterpret_cast<const V *>(psi + (base | seg[l >> L] | (l & segmask)));
-      *reinterpret_cast<V *>(tile + slot<R>(l)) = *src;
-    }
-    __syncthreads();
-    for (uint32_t s = 0; s < P.n_stages; ++s) {
-      const TileStage *st = stages + s;
-      if (st->kind == kStageRegisters) {
-        reg_stage<R, RB>(tile, t, base, st, rops, mats);
-      } else {
-        const TileOp *op = tops + st->begin;
-        const uint64_t gc = op->gcmask;
-        if ((base & gc) == gc) {
-          const amp_t<R> *m = mats + op->mat;
-          switch (op->k) {
-            case 1: tile_apply<R, 1>(tile, t, op, m); break;
-            case 2: tile_apply<R, 2>(tile, t, op, m); break;
-            case 3: tile_apply<R, 3>(tile, t, op, m); break;
-            case 4: tile_apply<R, 4>(tile, t, op, m); break;
-            default: tile_apply<R, 5>(tile, t, op, m); break;
-          }
-        }
+  fill_segments(P, seg);
+  for (uint64_t tix = blockIdx.x; tix < n_tiles; tix += gridDim.x)
+    tile_body<R, RB>(psi, tix, P, seg, tile, stages, rops, tops, mats);
+}
+
+// Many circuits at once: workgroup b runs tile (b - job.first_block) of job block_job[b]; every
+// job is one pass of one circuit whose state starts at psi + job.state_off.
+template <typename R, int RB>
+__global__ __launch_bounds__(kBlock) void k_batch_tile_pass(amp_t<R> *__restrict__ psi,
+                                                            const BatchJob *__restrict__ jobs,
+                                                            const uint32_t *__restrict__ block_job,
+                                                            const TileStage *__restrict__ stages,
+                                                            const RegOp *__restrict__ rops,
+                                                            const TileOp *__restrict__ tops,
+                                                            const amp_t<R> *__restrict__ mats) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  __shared__ uint64_t seg[1 << kTileMaxHigh];
+  amp_t<R> *tile = reinterpret_cast<amp_t<R> *>(smem);
+  const BatchJob *job = jobs + block_job[blockIdx.x];
+  const TilePass P = job->P;
+  fill_segments(P, seg);
+  tile_body<R, RB>(psi + job->state_off, blockIdx.x - job->first_block, P, seg, tile, stages + job->stage_off,
+                   rops + job->rop_off, tops + job->top_off, mats + job->mat_off);
+}
+
+// ------------------------------------------------------------------------------ batched circuits
+template <typename R>
+__global__ __launch_bounds__(kBlock) void k_batch_zero(amp_t<R> *__restrict__ psi, uint64_t n) {
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
+    psi[i] = make_amp<R>(0.0, 0.0);
+}
+template <typename R>
+__global__ void k_batch_ones(amp_t<R> *__restrict__ psi, const uint64_t *__restrict__ off, int n) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c < n) psi[off[c]] = make_amp<R>(1.0, 0.0);
+}
+
+template <typename R>
+__global__ __launch_bounds__(kBlock) void k_batch_energy(const amp_t<R> *__restrict__ psi,
+                                                         const EnergyJob *__restrict__ jobs,
+                                                         const uint32_t *__restrict__ block_job,
+                                                         const PauliTerm *__restrict__ terms,
+                                                         double *partials) {
+  __shared__ double lds[kBlock / 64];
+  const EnergyJob *job = jobs + block_job[blockIdx.x];
+  const amp_t<R> *st = psi + job->state_off;
+  const PauliTerm *tm = terms + job->term_off;
+  const int nt = (int)job->n_terms;
+  const uint64_t lo = (uint64_t)(blockIdx.x - job->first_block) * job->chunk;
+  const uint64_t hi = min(job->n_amps, lo + job->chunk);
+  Kahan acc;
+  for (uint64_t i = lo + threadIdx.x; i < hi; i += kBlock) {
+    const amp_t<R> a = st[i];
+    const double ar = re(a), ai = im(a);
+    double ur = 0.0, ui = 0.0;
+    for (int t = 0; t < nt; ++t) {
+      const PauliTerm pt = tm[t];
+      const uint64_t j = i ^ pt.x;
+      const amp_t<R> b = st[j];
+      double br = re(b), bi = im(b);
+      if (__popcll(j & pt.z) & 1) {
+        br = -br;
+        bi = -bi;
       }
-      __syncthreads();
+      ur += br * pt.c_re - bi * pt.c_im;
+      ui += br * pt.c_im + bi * pt.c_re;
     }
-    for (uint32_t l = threadIdx.x * VN; l < tsize; l += kBlock * VN) {
-      V *dst = reinterpret_cast<V *>(psi + (base | seg[l >> L] | (l & segmask)));
-      *dst = *reinterpret_cast<const V *>(tile + slot<R>(l));
-    }
-    __syncthreads();
+    acc.add(ar * ur + ai * ui);
   }
+  double t = block_sum(acc.value(), lds);
+  if (threadIdx.x == 0) partials[blockIdx.x] = t;
+}
+
+__global__ void k_batch_energy_finish(const EnergyJob *__restrict__ jobs, int n_jobs,
+                                      const double *__restrict__ partials, double *energies) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n_jobs) return;
+  double s = 0.0;
+  for (uint32_t b = 0; b < jobs[j].n_blocks; ++b) s += partials[jobs[j].first_block + b];
+  energies[j] = s;
 }
 
 // ------------------------------------------------------------------------------ reductions
@@ -622,6 +715,35 @@ __global__ __launch_bounds__(kBlock) void k_pauli_expect(const amp_t<R> *__restr
     partials[2 * blockIdx.x] = tr;
     partials[2 * blockIdx.x + 1] = ti;
   }
+}
+
+template <typename R>
+__global__ __launch_bounds__(kBlock) void k_pauli_energy(const amp_t<R> *__restrict__ psi, uint64_t n,
+                                                         const PauliTerm *__restrict__ terms,
+                                                         int n_terms, double *partials) {
+  __shared__ double lds[kBlock / 64];
+  Kahan acc;
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+    const amp_t<R> a = psi[i];
+    const double ar = re(a), ai = im(a);
+    double ur = 0.0, ui = 0.0;
+    for (int t = 0; t < n_terms; ++t) {
+      const PauliTerm pt = terms[t];
+      const uint64_t j = i ^ pt.x;
+      const amp_t<R> b = psi[j];
+      double br = re(b), bi = im(b);
+      if (__popcll(j & pt.z) & 1) {
+        br = -br;
+        bi = -bi;
+      }
+      ur += br * pt.c_re - bi * pt.c_im;
+      ui += br * pt.c_im + bi * pt.c_re;
+    }
+    acc.add(ar * ur + ai * ui);  // Re(conj(a) * u)
+  }
+  double t = block_sum(acc.value(), lds);
+  if (threadIdx.x == 0) partials[blockIdx.x] = t;
 }
 
 template <typename R, bool EVOLVE>
@@ -910,7 +1032,11 @@ void launch_tile_pass(amp_t<R> *psi, uint64_t n_tiles, const TilePass &P, const 
                                (int)(sizeof(amp_t<R>) << kTileMaxBits)));
     attr_set = true;
   }
-  const uint64_t grid = std::min<uint64_t>(n_tiles, 1u << 20);
+  static const uint64_t cap = [] {
+    const char *e = std::getenv("PROJECTQ_AMD_TILE_GRID");
+    return e ? std::max<uint64_t>(1, std::strtoull(e, nullptr, 10)) : (uint64_t(1) << 20);
+  }();
+  const uint64_t grid = std::min<uint64_t>(n_tiles, cap);
   hipLaunchKernelGGL((k_tile_pass<R, RB>), dim3((unsigned)grid), dim3(kBlock), lds, s, psi, n_tiles, P,
                      d_stages, d_rops, d_tops, d_mats);
   QS_HIP(hipGetLastError());
@@ -924,6 +1050,53 @@ void tile_pass(amp_t<R> *psi, uint64_t n_tiles, const TilePass &P, int rb, const
   if (rb == 5) launch_tile_pass<R, 5>(psi, n_tiles, P, d_stages, d_rops, d_tops, d_mats, s);
   else if (rb == 4) launch_tile_pass<R, 4>(psi, n_tiles, P, d_stages, d_rops, d_tops, d_mats, s);
   else throw ValueError("tile pass: register bits must be 4 or 5");
+}
+
+template <typename R>
+void batch_init(amp_t<R> *psi, uint64_t total, const uint64_t *state_off, int n_states, hipStream_t s) {
+  hipLaunchKernelGGL(k_batch_zero<R>, dim3(grid_for(total)), dim3(kBlock), 0, s, psi, total);
+  QS_HIP(hipGetLastError());
+  hipLaunchKernelGGL(k_batch_ones<R>, dim3((n_states + 255) / 256), dim3(256), 0, s, psi, state_off, n_states);
+  QS_HIP(hipGetLastError());
+}
+
+template <typename R, int RB>
+void launch_batch_tile_pass(amp_t<R> *psi, const BatchJob *jobs, const uint32_t *block_job, uint32_t n_blocks,
+                            uint32_t max_t, const TileStage *stages, const RegOp *rops, const TileOp *tops,
+                            const amp_t<R> *mats, hipStream_t s) {
+  static bool attr_set = false;
+  if (!attr_set) {
+    QS_HIP(hipFuncSetAttribute((const void *)k_batch_tile_pass<R, RB>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                               (int)(sizeof(amp_t<R>) << kTileMaxBits)));
+    attr_set = true;
+  }
+  hipLaunchKernelGGL((k_batch_tile_pass<R, RB>), dim3(n_blocks), dim3(kBlock), sizeof(amp_t<R>) << max_t, s, psi,
+                     jobs, block_job, stages, rops, tops, mats);
+  QS_HIP(hipGetLastError());
+}
+
+template <typename R>
+void batch_tile_pass(amp_t<R> *psi, const BatchJob *jobs, const uint32_t *block_job, uint32_t n_blocks,
+                     uint32_t max_t, int rb, const TileStage *stages, const RegOp *rops, const TileOp *tops,
+                     const amp_t<R> *mats, hipStream_t s) {
+  if (max_t > (uint32_t)kTileMaxBits || n_blocks == 0) throw ValueError("batch pass: bad launch shape");
+  if (rb == 5) launch_batch_tile_pass<R, 5>(psi, jobs, block_job, n_blocks, max_t, stages, rops, tops, mats, s);
+  else if (rb == 4) launch_batch_tile_pass<R, 4>(psi, jobs, block_job, n_blocks, max_t, stages, rops, tops, mats, s);
+  else throw ValueError("batch pass: register bits must be 4 or 5");
+}
+
+template <typename R>
+void batch_energy(const amp_t<R> *psi, const EnergyJob *jobs, const uint32_t *block_job, uint32_t n_blocks,
+                  const PauliTerm *terms, double *partials, hipStream_t s) {
+  hipLaunchKernelGGL(k_batch_energy<R>, dim3(n_blocks), dim3(kBlock), 0, s, psi, jobs, block_job, terms, partials);
+  QS_HIP(hipGetLastError());
+}
+
+void batch_energy_finish(const EnergyJob *jobs, int n_jobs, const double *partials, double *energies,
+                         hipStream_t s) {
+  hipLaunchKernelGGL(k_batch_energy_finish, dim3((n_jobs + 255) / 256), dim3(256), 0, s, jobs, n_jobs, partials,
+                     energies);
+  QS_HIP(hipGetLastError());
 }
 
 template <typename R>
@@ -986,6 +1159,14 @@ template <typename R>
 void pauli_expect(const amp_t<R> *psi, uint64_t n, uint64_t x, uint64_t z, double *partials,
                   hipStream_t s) {
   hipLaunchKernelGGL(k_pauli_expect<R>, dim3(kReduceGrid), dim3(kBlock), 0, s, psi, n, x, z,
+                     partials);
+  QS_HIP(hipGetLastError());
+}
+
+template <typename R>
+void pauli_energy(const amp_t<R> *psi, uint64_t n, const PauliTerm *terms, int n_terms, double *partials,
+                  hipStream_t s) {
+  hipLaunchKernelGGL(k_pauli_energy<R>, dim3(kReduceGrid), dim3(kBlock), 0, s, psi, n, terms, n_terms,
                      partials);
   QS_HIP(hipGetLastError());
 }
@@ -1061,6 +1242,12 @@ void math_accumulate(const amp_t<R> *src, amp_t<R> *dst, uint64_t n, const MathA
 
 // ------------------------------------------------------------------------------ instantiations
 #define QS_INSTANTIATE(R)                                                                          \
+  template void batch_init<R>(amp_t<R> *, uint64_t, const uint64_t *, int, hipStream_t);          \
+  template void batch_tile_pass<R>(amp_t<R> *, const BatchJob *, const uint32_t *, uint32_t,       \
+                                   uint32_t, int, const TileStage *, const RegOp *, const TileOp *,\
+                                   const amp_t<R> *, hipStream_t);                                \
+  template void batch_energy<R>(const amp_t<R> *, const EnergyJob *, const uint32_t *, uint32_t,   \
+                                const PauliTerm *, double *, hipStream_t);                        \
   template void tile_pass<R>(amp_t<R> *, uint64_t, const TilePass &, int, const TileStage *,      \
                              const RegOp *, const TileOp *, const amp_t<R> *, hipStream_t);       \
   template void apply_block<R>(amp_t<R> *, uint64_t, const BitInsert &, uint64_t,                 \
@@ -1076,6 +1263,8 @@ void math_accumulate(const amp_t<R> *src, amp_t<R> *dst, uint64_t n, const MathA
                               hipStream_t);                                                       \
   template void drop_bit<R>(const amp_t<R> *, amp_t<R> *, uint64_t, int, int, hipStream_t);       \
   template void pauli_expect<R>(const amp_t<R> *, uint64_t, uint64_t, uint64_t, double *,         \
+                                hipStream_t);                                                     \
+  template void pauli_energy<R>(const amp_t<R> *, uint64_t, const PauliTerm *, int, double *,      \
                                 hipStream_t);                                                     \
   template void pauli_sum<R>(const amp_t<R> *, amp_t<R> *, amp_t<R> *, uint64_t,                  \
                              const PauliTerm *, int, double, double, bool, uint64_t, double *,    \

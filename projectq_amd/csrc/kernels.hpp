@@ -32,6 +32,23 @@ template <typename R>
 void tile_pass(amp_t<R> *psi, uint64_t n_tiles, const TilePass &P, int rb, const TileStage *d_stages,
                const RegOp *d_rops, const TileOp *d_tops, const amp_t<R> *d_mats, hipStream_t s);
 
+// ---- batched circuits (one state per circuit in one buffer) ----
+// every state <- |0...0>: zero the whole buffer, then amplitude state_off[c] = 1 (no memset, so
+// the sequence can be captured in a graph)
+template <typename R>
+void batch_init(amp_t<R> *psi, uint64_t total_amps, const uint64_t *state_off, int n_states, hipStream_t s);
+// one pass of every circuit: n_blocks workgroups, block_job maps each to its job
+template <typename R>
+void batch_tile_pass(amp_t<R> *psi, const BatchJob *jobs, const uint32_t *block_job, uint32_t n_blocks,
+                     uint32_t max_t, int rb, const TileStage *stages, const RegOp *rops, const TileOp *tops,
+                     const amp_t<R> *mats, hipStream_t s);
+// partials[b] for every energy workgroup, then energies[j] = sum of job j's partials in order
+template <typename R>
+void batch_energy(const amp_t<R> *psi, const EnergyJob *jobs, const uint32_t *block_job, uint32_t n_blocks,
+                  const PauliTerm *terms, double *partials, hipStream_t s);
+void batch_energy_finish(const EnergyJob *jobs, int n_jobs, const double *partials, double *energies,
+                         hipStream_t s);
+
 // dense gate block: psi[base + off(r)] = sum_c M[r][c] psi[base + off(c)] for every base with
 // target bits 0 and control bits 1.  tpos[l] is the position of matrix bit l; ins lists the
 // sorted target+control positions; n_groups = 2^(n - k - #controls).
@@ -78,6 +95,11 @@ void drop_bit(const amp_t<R> *src, amp_t<R> *dst, uint64_t n_out, int p, int val
 // partials[2*b + {0,1}] <- block sums of conj(psi_i) * (-1)^popcount((i^x)&z) * psi_{i^x}
 template <typename R>
 void pauli_expect(const amp_t<R> *psi, uint64_t n_amps, uint64_t x, uint64_t z,
+                  double *partials, hipStream_t s);
+// partials[b] <- block sums of Re( conj(psi_i) * sum_t c_t (-1)^popcount((i^x_t)&z_t) psi_{i^x_t} )
+// (all terms in one pass; c_t already carries the string's phase omega_t)
+template <typename R>
+void pauli_energy(const amp_t<R> *psi, uint64_t n_amps, const PauliTerm *terms, int n_terms,
                   double *partials, hipStream_t s);
 // dst_i <- f * sum_t c_t (-1)^popcount((i^x_t)&z_t) src_{i^x_t}; with `evolve` also
 // out_i += dst_i and nrm partial sums of |dst_i|^2 on entries with (i & cmask) == cmask

@@ -74,6 +74,26 @@ struct TilePass {
   BitInsert tiles;              // all tile positions (global, ascending): enumerates tile bases
 };
 
+// One pass of one circuit inside a batched launch (k_batch_tile_pass).  Offsets index the
+// batch-wide arrays of stages, register ops, LDS ops and matrices.
+struct BatchJob {
+  uint64_t state_off;    // first amplitude of this circuit's state
+  uint32_t first_block;  // first workgroup of this job in the launch
+  uint32_t stage_off, rop_off, top_off, mat_off;
+  uint32_t pad;
+  TilePass P;
+};
+
+// Expectation value of one circuit inside a batched launch (k_batch_energy): workgroups
+// first_block .. first_block + n_blocks - 1 each cover `chunk` amplitudes.
+struct EnergyJob {
+  uint64_t state_off;
+  uint64_t n_amps;
+  uint64_t chunk;
+  uint32_t first_block, n_blocks;
+  uint32_t term_off, n_terms;
+};
+
 // LDS address of tile-local index x: an XOR swizzle of the low bits by the next bits, so that
 // threads reading with power-of-two strides spread over the banks.  Pairs (2j, 2j+1) stay
 // adjacent for complex64 so 16-byte vector accesses still work.  A bijection on [0, 2^t).

@@ -313,4 +313,24 @@ void build_tile_pass(const std::vector<PGate> &gates, const PPass &ps, unsigned 
   P.n_stages = (uint32_t)prog.stages.size();
 }
 
+void encode_pauli(const std::vector<std::pair<uint32_t, char>> &ops, uint64_t *x, uint64_t *z,
+                  std::complex<double> *omega) {
+  uint64_t X = 0, Z = 0;
+  std::complex<double> w(1.0, 0.0);
+  for (const auto &op : ops) {
+    if (op.first >= 64) throw std::invalid_argument("Pauli string position out of range");
+    const uint64_t q = uint64_t(1) << op.first;
+    const double sgn = (X & q) ? -1.0 : 1.0;
+    switch (op.second) {
+      case 'X': X ^= q; break;
+      case 'Z': w *= sgn; Z ^= q; break;
+      case 'Y': w *= std::complex<double>(0.0, sgn); X ^= q; Z ^= q; break;
+      default: throw std::invalid_argument("Pauli operator must be X, Y or Z");
+    }
+  }
+  *x = X;
+  *z = Z;
+  *omega = w;
+}
+
 }  // namespace qs

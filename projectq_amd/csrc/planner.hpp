@@ -78,6 +78,12 @@ struct TileProgram {
 void build_tile_pass(const std::vector<PGate> &gates, const PPass &pass, unsigned t, unsigned rb,
                      TileProgram &prog);
 
+// A Pauli string applied left to right, ops given as (bit position, 'X'|'Y'|'Z'), equals
+// omega * X^x Z^z:  X_q: x ^= q;  Z_q: omega *= (-1)^{x_q}, z ^= q;  Y_q = i X_q Z_q:
+// omega *= i (-1)^{x_q}, x ^= q, z ^= q.  Then (P psi)_i = omega (-1)^{popcount((i^x)&z)} psi_{i^x}.
+void encode_pauli(const std::vector<std::pair<uint32_t, char>> &ops, uint64_t *x, uint64_t *z,
+                  std::complex<double> *omega);
+
 // Default slot choice: highest lookahead score first.
 std::vector<uint32_t> choose_by_score(const std::vector<SlotCandidate> &cands, unsigned slots);
 
