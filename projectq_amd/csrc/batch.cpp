@@ -39,6 +39,7 @@ Batch::Batch(bool c64, std::vector<BatchCircuit> circuits, bool autotune) : c64_
     for (auto &t : c.terms)
       if ((t.x | t.z) >> c.n) throw ValueError("batch: Pauli string outside its circuit");
   }
+  dev::prepare_tile_kernels();
   QS_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
   QS_HIP(hipEventCreate(&ev0_));
   QS_HIP(hipEventCreate(&ev1_));

@@ -1025,16 +1025,11 @@ void apply_block(amp_t<R> *psi, uint64_t n_groups, const BitInsert &ins, uint64_
 template <typename R, int RB>
 void launch_tile_pass(amp_t<R> *psi, uint64_t n_tiles, const TilePass &P, const TileStage *d_stages,
                       const RegOp *d_rops, const TileOp *d_tops, const amp_t<R> *d_mats, hipStream_t s) {
-  static bool attr_set = false;
+  prepare_tile_kernels();
   const size_t lds = sizeof(amp_t<R>) << P.t;
-  if (!attr_set) {
-    QS_HIP(hipFuncSetAttribute((const void *)k_tile_pass<R, RB>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                               (int)(sizeof(amp_t<R>) << kTileMaxBits)));
-    attr_set = true;
-  }
   static const uint64_t cap = [] {
     const char *e = std::getenv("PROJECTQ_AMD_TILE_GRID");
-    return e ? std::max<uint64_t>(1, std::strtoull(e, nullptr, 10)) : (uint64_t(1) << 20);
+    return e ? std::max<uint64_t>(1, std::strtoull(e, nullptr, 10)) : uint64_t(8192);
   }();
   const uint64_t grid = std::min<uint64_t>(n_tiles, cap);
   hipLaunchKernelGGL((k_tile_pass<R, RB>), dim3((unsigned)grid), dim3(kBlock), lds, s, psi, n_tiles, P,
@@ -1052,6 +1047,20 @@ void tile_pass(amp_t<R> *psi, uint64_t n_tiles, const TilePass &P, int rb, const
   else throw ValueError("tile pass: register bits must be 4 or 5");
 }
 
+void prepare_tile_kernels() {
+  static bool done = false;
+  if (done) return;
+  QS_HIP(hipFuncSetAttribute((const void *)k_tile_pass<float, 4>, hipFuncAttributeMaxDynamicSharedMemorySize, 8 << kTileMaxBits));
+  QS_HIP(hipFuncSetAttribute((const void *)k_tile_pass<float, 5>, hipFuncAttributeMaxDynamicSharedMemorySize, 8 << kTileMaxBits));
+  QS_HIP(hipFuncSetAttribute((const void *)k_tile_pass<double, 4>, hipFuncAttributeMaxDynamicSharedMemorySize, 16 << kTileMaxBits));
+  QS_HIP(hipFuncSetAttribute((const void *)k_tile_pass<double, 5>, hipFuncAttributeMaxDynamicSharedMemorySize, 16 << kTileMaxBits));
+  QS_HIP(hipFuncSetAttribute((const void *)k_batch_tile_pass<float, 4>, hipFuncAttributeMaxDynamicSharedMemorySize, 8 << kTileMaxBits));
+  QS_HIP(hipFuncSetAttribute((const void *)k_batch_tile_pass<float, 5>, hipFuncAttributeMaxDynamicSharedMemorySize, 8 << kTileMaxBits));
+  QS_HIP(hipFuncSetAttribute((const void *)k_batch_tile_pass<double, 4>, hipFuncAttributeMaxDynamicSharedMemorySize, 16 << kTileMaxBits));
+  QS_HIP(hipFuncSetAttribute((const void *)k_batch_tile_pass<double, 5>, hipFuncAttributeMaxDynamicSharedMemorySize, 16 << kTileMaxBits));
+  done = true;
+}
+
 template <typename R>
 void batch_init(amp_t<R> *psi, uint64_t total, const uint64_t *state_off, int n_states, hipStream_t s) {
   hipLaunchKernelGGL(k_batch_zero<R>, dim3(grid_for(total)), dim3(kBlock), 0, s, psi, total);
@@ -1064,12 +1073,7 @@ template <typename R, int RB>
 void launch_batch_tile_pass(amp_t<R> *psi, const BatchJob *jobs, const uint32_t *block_job, uint32_t n_blocks,
                             uint32_t max_t, const TileStage *stages, const RegOp *rops, const TileOp *tops,
                             const amp_t<R> *mats, hipStream_t s) {
-  static bool attr_set = false;
-  if (!attr_set) {
-    QS_HIP(hipFuncSetAttribute((const void *)k_batch_tile_pass<R, RB>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                               (int)(sizeof(amp_t<R>) << kTileMaxBits)));
-    attr_set = true;
-  }
+  prepare_tile_kernels();
   hipLaunchKernelGGL((k_batch_tile_pass<R, RB>), dim3(n_blocks), dim3(kBlock), sizeof(amp_t<R>) << max_t, s, psi,
                      jobs, block_job, stages, rops, tops, mats);
   QS_HIP(hipGetLastError());

@@ -32,6 +32,9 @@ template <typename R>
 void tile_pass(amp_t<R> *psi, uint64_t n_tiles, const TilePass &P, int rb, const TileStage *d_stages,
                const RegOp *d_rops, const TileOp *d_tops, const amp_t<R> *d_mats, hipStream_t s);
 
+// set the dynamic-LDS limit of every tile-pass kernel (call before any graph capture)
+void prepare_tile_kernels();
+
 // ---- batched circuits (one state per circuit in one buffer) ----
 // every state <- |0...0>: zero the whole buffer, then amplitude state_off[c] = 1 (no memset, so
 // the sequence can be captured in a graph)
