@@ -163,6 +163,31 @@ int qs_reset_stats(qs_simulator *sim);
  * (precisions may differ); accumulated in FP64.  result = (re, im). */
 int qs_overlap(qs_simulator *a, qs_simulator *b, double *result);
 
+/* ---- planner controls and the learned cost model (extensions, no counterpart in the reference) --
+ * Tile shape of QS_PLANNER_TILE: tile bits (6..13; 12 for complex128), register bits (4 or 5) and
+ * low contiguous bits every tile keeps. */
+int qs_set_tile_params(qs_simulator *sim, uint32_t tile_bits, uint32_t reg_bits, uint32_t low_bits);
+/* Record every window pass: QS_PASS_FEATURES features and its GPU time. */
+#define QS_PASS_FEATURES 24
+int qs_set_recording(qs_simulator *sim, int enabled);
+/* Move up to `cap` records out (features row-major, ms); *rows = how many.  Synchronises. */
+int qs_take_records(qs_simulator *sim, float *features, float *ms, size_t cap, size_t *rows);
+/* Load (n_weights = qs_mlp_num_weights()) or clear (n_weights = 0) the model QS_PLANNER_LEARNED
+ * uses; without a model the learned planner falls back to greedy max-5 blocks. */
+int qs_set_planner_model(qs_simulator *sim, const float *weights, size_t n_weights);
+/* The last window's plan: kind (0 greedy blocks, 1 tile passes), tile shape, passes, number of
+ * candidate plans compared and the model's predicted time (ms). */
+int qs_last_plan(qs_simulator *sim, int *kind, uint32_t *tile_bits, uint32_t *reg_bits,
+                 uint32_t *low_bits, uint32_t *passes, uint32_t *candidates, double *predicted_ms);
+/* The MLP (24 -> 64 -> 64 -> 1, ReLU, fp32) on the GPU: weight count, inference, loss and
+ * gradient (mean squared error), and Adam training on minibatches (w_init may be NULL). */
+size_t qs_mlp_num_weights(void);
+int qs_mlp_predict(const float *weights, const float *features, size_t rows, float *out);
+int qs_mlp_loss_grad(const float *weights, const float *features, const float *targets, size_t rows,
+                     float *loss, float *grad);
+int qs_mlp_train(const float *features, const float *targets, size_t rows, int epochs, int batch,
+                 float lr, uint32_t seed, const float *w_init, float *w_out, float *final_loss);
+
 /* ---- batched circuits (extension, no counterpart in the reference) ---------------------------
  * B independent circuits of uneven qubit counts, each on its own register starting in |0..0>,
  * run together on one GPU and reduced to <psi_b|H_b|psi_b>.  Used by variational workloads.

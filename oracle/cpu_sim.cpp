@@ -829,6 +829,36 @@ API int qs_overlap(qs_simulator *a, qs_simulator *b, double *out) {
   });
 }
 
+// Planner controls and the MLP are GPU-side features with no reference counterpart; the oracle
+// only answers for the simulator surface.
+API int qs_set_tile_params(qs_simulator *, uint32_t, uint32_t, uint32_t) { return QS_OK; }
+API int qs_set_recording(qs_simulator *, int) { return QS_OK; }
+API int qs_take_records(qs_simulator *, float *, float *, size_t, size_t *rows) {
+  *rows = 0;
+  return QS_OK;
+}
+API int qs_set_planner_model(qs_simulator *, const float *, size_t) { return QS_OK; }
+API int qs_last_plan(qs_simulator *, int *kind, uint32_t *t, uint32_t *rb, uint32_t *low, uint32_t *p, uint32_t *c,
+                     double *pred) {
+  *kind = -1;
+  *t = *rb = *low = *p = *c = 0;
+  *pred = 0.0;
+  return QS_OK;
+}
+API size_t qs_mlp_num_weights(void) { return 0; }
+API int qs_mlp_predict(const float *, const float *, size_t, float *) {
+  g_last_error = "the oracle has no MLP (see oracle/mlp.py)";
+  return QS_ERR_RUNTIME;
+}
+API int qs_mlp_loss_grad(const float *, const float *, const float *, size_t, float *, float *) {
+  g_last_error = "the oracle has no MLP (see oracle/mlp.py)";
+  return QS_ERR_RUNTIME;
+}
+API int qs_mlp_train(const float *, const float *, size_t, int, int, float, uint32_t, const float *, float *, float *) {
+  g_last_error = "the oracle has no MLP (see oracle/mlp.py)";
+  return QS_ERR_RUNTIME;
+}
+
 // Batched circuits, restated one circuit at a time: gates applied unfused in order, then the
 // expectation value of simulator.hpp:292-322.
 struct qs_batch {

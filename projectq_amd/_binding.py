@@ -103,6 +103,48 @@ _SIGNATURES = {
     ),
     "qs_reset_stats": (ctypes.c_int, [ctypes.c_void_p]),
     "qs_overlap": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, _f64p]),
+    "qs_set_tile_params": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32]),
+    "qs_set_recording": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
+    "qs_take_records": (
+        ctypes.c_int,
+        [ctypes.c_void_p, ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float), ctypes.c_size_t, _szp],
+    ),
+    "qs_set_planner_model": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_float), ctypes.c_size_t]),
+    "qs_last_plan": (
+        ctypes.c_int,
+        [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int), _u32p, _u32p, _u32p, _u32p, _u32p, _f64p],
+    ),
+    "qs_mlp_num_weights": (ctypes.c_size_t, []),
+    "qs_mlp_predict": (
+        ctypes.c_int,
+        [ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float), ctypes.c_size_t, ctypes.POINTER(ctypes.c_float)],
+    ),
+    "qs_mlp_loss_grad": (
+        ctypes.c_int,
+        [
+            ctypes.POINTER(ctypes.c_float),
+            ctypes.POINTER(ctypes.c_float),
+            ctypes.POINTER(ctypes.c_float),
+            ctypes.c_size_t,
+            ctypes.POINTER(ctypes.c_float),
+            ctypes.POINTER(ctypes.c_float),
+        ],
+    ),
+    "qs_mlp_train": (
+        ctypes.c_int,
+        [
+            ctypes.POINTER(ctypes.c_float),
+            ctypes.POINTER(ctypes.c_float),
+            ctypes.c_size_t,
+            ctypes.c_int,
+            ctypes.c_int,
+            ctypes.c_float,
+            ctypes.c_uint32,
+            ctypes.POINTER(ctypes.c_float),
+            ctypes.POINTER(ctypes.c_float),
+            ctypes.POINTER(ctypes.c_float),
+        ],
+    ),
     "qs_batch_create": (
         ctypes.c_int,
         [
@@ -130,6 +172,8 @@ _SIGNATURES = {
     ),
     "qs_batch_destroy": (None, [ctypes.c_void_p]),
 }
+
+PASS_FEATURES = 24
 
 #: every symbol the C ABI declares (include/projectq_amd.h)
 ABI_SYMBOLS = tuple(_SIGNATURES)
@@ -485,6 +529,58 @@ class CAbiSimulator:
         self._check(self._lib.qs_overlap(self._handle, other._handle, _ptr(out, ctypes.c_double)))
         return complex(out[0], out[1])
 
+    # ------------------------------------------------------------------ planner controls
+    def set_tile_params(self, tile_bits, reg_bits, low_bits):
+        self._check(self._lib.qs_set_tile_params(self._handle, int(tile_bits), int(reg_bits), int(low_bits)))
+
+    def set_recording(self, enabled=True):
+        self._check(self._lib.qs_set_recording(self._handle, 1 if enabled else 0))
+
+    def take_records(self, cap=1 << 16):
+        """(features [rows, PASS_FEATURES] float32, pass times in ms) of the recorded passes."""
+        feats = np.zeros((cap, PASS_FEATURES), dtype=np.float32)
+        ms = np.zeros(cap, dtype=np.float32)
+        rows = ctypes.c_size_t()
+        self._check(
+            self._lib.qs_take_records(
+                self._handle, _ptr(feats, ctypes.c_float), _ptr(ms, ctypes.c_float), cap, ctypes.byref(rows)
+            )
+        )
+        return feats[: rows.value].copy(), ms[: rows.value].copy()
+
+    def set_planner_model(self, weights):
+        if weights is None:
+            self._check(self._lib.qs_set_planner_model(self._handle, None, 0))
+            return
+        w = np.ascontiguousarray(np.asarray(weights, dtype=np.float32).reshape(-1))
+        self._check(self._lib.qs_set_planner_model(self._handle, _ptr(w, ctypes.c_float), len(w)))
+
+    def last_plan(self):
+        kind = ctypes.c_int()
+        t, rb, low, passes, cands = (ctypes.c_uint32() for _ in range(5))
+        pred = ctypes.c_double()
+        self._check(
+            self._lib.qs_last_plan(
+                self._handle,
+                ctypes.byref(kind),
+                ctypes.byref(t),
+                ctypes.byref(rb),
+                ctypes.byref(low),
+                ctypes.byref(passes),
+                ctypes.byref(cands),
+                ctypes.byref(pred),
+            )
+        )
+        return {
+            "kind": {0: "greedy_blocks", 1: "tile"}.get(kind.value, "none"),
+            "tile_bits": t.value,
+            "reg_bits": rb.value,
+            "low_bits": low.value,
+            "passes": passes.value,
+            "candidates": cands.value,
+            "predicted_ms": pred.value,
+        }
+
     def cheat_numpy(self):
         """cheat() without the Python-list conversion: (dict id->position, complex128 ndarray)."""
         nq = ctypes.c_size_t()
@@ -601,6 +697,54 @@ class CAbiBatch:
         if handle is not None and handle.value:
             self._lib.qs_batch_destroy(handle)
             self._handle = None
+
+
+def _f32(a):
+    return np.ascontiguousarray(np.asarray(a, dtype=np.float32))
+
+
+class MLP:
+    """The planner's cost model on the GPU (qs_mlp_*): 24 -> 64 -> 64 -> 1, ReLU, fp32."""
+
+    def __init__(self, lib):
+        self._lib = lib
+        self.n_weights = int(lib.qs_mlp_num_weights())
+
+    def _check(self, status):
+        if status != QS_OK:
+            msg = (self._lib.qs_last_error() or b"").decode(errors="replace")
+            raise (ValueError if status == QS_ERR_VALUE else RuntimeError)(msg)
+
+    def predict(self, weights, features):
+        w, x = _f32(weights).reshape(-1), _f32(features).reshape(-1, PASS_FEATURES)
+        out = np.zeros(len(x), dtype=np.float32)
+        self._check(self._lib.qs_mlp_predict(_ptr(w, ctypes.c_float), _ptr(x, ctypes.c_float), len(x), _ptr(out, ctypes.c_float)))
+        return out
+
+    def loss_grad(self, weights, features, targets):
+        w, x, t = _f32(weights).reshape(-1), _f32(features).reshape(-1, PASS_FEATURES), _f32(targets).reshape(-1)
+        loss = ctypes.c_float()
+        grad = np.zeros(self.n_weights, dtype=np.float32)
+        self._check(
+            self._lib.qs_mlp_loss_grad(
+                _ptr(w, ctypes.c_float), _ptr(x, ctypes.c_float), _ptr(t, ctypes.c_float), len(x), ctypes.byref(loss),
+                _ptr(grad, ctypes.c_float),
+            )
+        )
+        return loss.value, grad
+
+    def train(self, features, targets, epochs=200, batch=256, lr=1e-3, seed=1, w_init=None):
+        x, t = _f32(features).reshape(-1, PASS_FEATURES), _f32(targets).reshape(-1)
+        w_out = np.zeros(self.n_weights, dtype=np.float32)
+        loss = ctypes.c_float()
+        wi = None if w_init is None else _f32(w_init).reshape(-1)
+        self._check(
+            self._lib.qs_mlp_train(
+                _ptr(x, ctypes.c_float), _ptr(t, ctypes.c_float), len(x), int(epochs), int(batch), float(lr), int(seed),
+                None if wi is None else _ptr(wi, ctypes.c_float), _ptr(w_out, ctypes.c_float), ctypes.byref(loss),
+            )
+        )
+        return w_out, loss.value
 
 
 def qubit_count_for(n_amps):

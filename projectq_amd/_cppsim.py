@@ -36,8 +36,28 @@ def library():
     return _lib
 
 
+MODEL_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "planner_mlp.npz")
+_model = None
+
+
+def planner_model():
+    """Weights of the learned planner's cost model (projectq_amd/planner_mlp.npz), or None."""
+    global _model
+    if _model is None and os.path.exists(MODEL_PATH):
+        import numpy as np
+
+        with np.load(MODEL_PATH, allow_pickle=False) as f:
+            _model = np.asarray(f["weights"], dtype=np.float32)
+    return _model
+
+
 class Simulator(CAbiSimulator):
     """State-vector simulator resident in one GPU's HBM, with the `_cppsim.Simulator` surface."""
 
-    def __init__(self, seed, precision="c128", fusion="reference", planner="greedy", capacity_qubits=0):
+    def __init__(self, seed, precision="c128", fusion="reference", planner="greedy", capacity_qubits=0,
+                 planner_weights=None):
         super().__init__(library(), seed, precision, fusion, planner, capacity_qubits)
+        if planner == "learned":
+            w = planner_weights if planner_weights is not None else planner_model()
+            if w is not None:
+                self.set_planner_model(w)

@@ -10,6 +10,7 @@
 
 #include "batch.hpp"
 #include "engine.hpp"
+#include "mlp.hpp"
 
 struct qs_simulator {
   qs::Engine engine;
@@ -302,4 +303,44 @@ API void qs_batch_destroy(qs_batch *b) {
   if (!b) return;
   (void)hipSetDevice(b->device);
   delete b;
+}
+
+API int qs_set_tile_params(qs_simulator *s, uint32_t t, uint32_t rb, uint32_t low) {
+  return guarded(s, [&] { s->engine.set_tile_params(t, rb, low); });
+}
+API int qs_set_recording(qs_simulator *s, int on) {
+  return guarded(s, [&] { s->engine.set_recording(on != 0); });
+}
+API int qs_take_records(qs_simulator *s, float *features, float *ms, size_t cap, size_t *rows) {
+  return guarded(s, [&] { *rows = s->engine.take_records(features, ms, cap); });
+}
+API int qs_set_planner_model(qs_simulator *s, const float *w, size_t n) {
+  return guarded(s, [&] {
+    if (n != 0 && n != (size_t)qs::mlp::kWeights) throw qs::ValueError("planner model: wrong number of weights");
+    s->engine.set_model(std::vector<float>(w, w + n));
+  });
+}
+API int qs_last_plan(qs_simulator *s, int *kind, uint32_t *t, uint32_t *rb, uint32_t *low, uint32_t *passes,
+                     uint32_t *cands, double *pred) {
+  return guarded(s, [&] {
+    const auto &p = s->engine.last_plan();
+    *kind = p.kind;
+    *t = p.tmax;
+    *rb = p.rb;
+    *low = p.lmin;
+    *passes = p.passes;
+    *cands = p.candidates;
+    *pred = p.predicted_ms;
+  });
+}
+API size_t qs_mlp_num_weights(void) { return (size_t)qs::mlp::kWeights; }
+API int qs_mlp_predict(const float *w, const float *X, size_t rows, float *out) {
+  return guarded(nullptr, [&] { qs::mlp::predict(w, X, rows, out); });
+}
+API int qs_mlp_loss_grad(const float *w, const float *X, const float *T, size_t rows, float *loss, float *grad) {
+  return guarded(nullptr, [&] { qs::mlp::loss_grad(w, X, T, rows, loss, grad); });
+}
+API int qs_mlp_train(const float *X, const float *T, size_t rows, int epochs, int batch, float lr, uint32_t seed,
+                     const float *w_init, float *w_out, float *final_loss) {
+  return guarded(nullptr, [&] { *final_loss = qs::mlp::train(X, T, rows, epochs, batch, lr, seed, w_init, w_out); });
 }

@@ -46,6 +46,7 @@ Engine::Engine(uint32_t seed, int precision, int fusion_mode, int planner, unsig
   // complex128: t = 12 (64 KiB, two workgroups per CU), rb = 4.
   tile_tmax_ = 12u;
   tile_rb_ = 4u;
+  tile_lmin_ = c64_ ? 5u : 4u;
   if (const char *env = std::getenv("PROJECTQ_AMD_TILE")) {
     unsigned t = 0, rb = 0;
     if (std::sscanf(env, "%u:%u", &t, &rb) != 2 || t < 6 || t > (c64_ ? 13u : 12u) || (rb != 4 && rb != 5))
@@ -79,6 +80,10 @@ Engine::~Engine() {
   for (auto &p : ev_used_) {
     (void)hipEventDestroy(p.first);
     (void)hipEventDestroy(p.second);
+  }
+  for (auto &r : records_) {
+    (void)hipEventDestroy(r.a);
+    (void)hipEventDestroy(r.b);
   }
   for (auto e : ev_free_) (void)hipEventDestroy(e);
   (void)hipFree(psi_);
@@ -349,44 +354,29 @@ void Engine::run() {
 
 // Greedy planner over a window of queued gates: the reference's block rule (min 4 / max 5,
 // simulator.hpp:204-222) applied gate by gate; each block is one streaming pass.
-void Engine::flush_window() {
-  if (window_.empty()) return;
-  for (const Gate &g : window_)
-    if (g.ids.empty() || g.ids.size() > 5) throw ValueError("Gates with more than 5 qubits are not supported!");
-  std::vector<Gate> gates;
-  gates.swap(window_);
-  if (planner_kind_ != 0) {
-    flush_window_tiles(gates);
-    return;
-  }
+std::vector<Block> Engine::greedy_blocks(const std::vector<Gate> &gates) const {
+  std::vector<Block> out;
   GateQueue q;
   for (const Gate &g : gates) {
     GateQueue trial = q;
     trial.insert(g);
     const unsigned now = trial.num_qubits(), before = q.num_qubits();
     if (now >= 4 && now <= 5) {
-      exec_block(trial.fuse());
+      out.push_back(trial.fuse());
       q.clear();
     } else if (now > 5 || (size_t)(now - g.ids.size()) > (size_t)before) {
-      if (!q.empty()) exec_block(q.fuse());
+      if (!q.empty()) out.push_back(q.fuse());
       q.clear();
       q.insert(g);
     } else {
       q = std::move(trial);
     }
   }
-  if (!q.empty()) exec_block(q.fuse());
+  if (!q.empty()) out.push_back(q.fuse());
+  return out;
 }
 
-// Tile size: 2^t amplitudes of LDS per workgroup (64 KiB), at most n bits, and small enough
-// that a state of n qubits still spreads over >= 256 tiles when n is large.
-unsigned Engine::tile_bits() const {
-  const unsigned tmax = tile_tmax_;
-  if (n_ <= 10) return n_;
-  return std::min(tmax, std::max(10u, n_ - 8));
-}
-
-void Engine::flush_window_tiles(std::vector<Gate> &gates) {
+std::vector<PGate> Engine::resolve(const std::vector<Gate> &gates) const {
   std::vector<PGate> pg;
   pg.reserve(gates.size());
   for (const Gate &g : gates) {
@@ -397,17 +387,129 @@ void Engine::flush_window_tiles(std::vector<Gate> &gates) {
     if (p.tmask() & p.cmask) continue;  // a control that is also a target: never applies
     pg.push_back(std::move(p));
   }
-  pg = merge_single_qubit_runs(pg);
-  TileShape shape{n_, tile_bits(), c64_ ? 5u : 4u};
-  std::vector<PPass> passes = schedule_tiles(pg, shape);
-  for (const PPass &ps : passes) exec_tile_pass(pg, ps, shape.t);
+  return merge_single_qubit_runs(pg);
 }
 
-void Engine::exec_tile_pass(const std::vector<PGate> &gates, const PPass &ps, unsigned t) {
-  TileProgram prog;
-  const unsigned rb = tile_rb_;
-  build_tile_pass(gates, ps, t, rb, prog);
+// Tile size: 2^t amplitudes of LDS per workgroup, at most n bits, and small enough that a state
+// of n qubits still spreads over >= 256 tiles when n is large.
+unsigned Engine::tile_bits(unsigned tmax) const {
+  if (n_ <= 10) return n_;
+  return std::min(tmax, std::max(10u, n_ - 8));
+}
+
+std::vector<TileProgram> Engine::plan_tiles(const std::vector<PGate> &pg, unsigned tmax, unsigned rb,
+                                            unsigned lmin, unsigned *t_used) const {
+  const unsigned t = tile_bits(tmax);
+  *t_used = t;
+  std::vector<PPass> passes = schedule_tiles(pg, TileShape{n_, t, lmin});
+  std::vector<TileProgram> progs(passes.size());
+  for (size_t i = 0; i < passes.size(); ++i) build_tile_pass(pg, passes[i], t, rb, progs[i]);
+  return progs;
+}
+
+PassFeatures Engine::block_features(const Block &b) const {
+  unsigned lo = 64;
+  for (unsigned q : b.qubits) lo = std::min(lo, pos_.at(q));
+  return block_pass_features((unsigned)b.qubits.size(), (unsigned)b.ctrl.size(), lo, n_, c64_);
+}
+
+void Engine::flush_window() {
+  if (window_.empty()) return;
+  for (const Gate &g : window_)
+    if (g.ids.empty() || g.ids.size() > 5) throw ValueError("Gates with more than 5 qubits are not supported!");
+  std::vector<Gate> gates;
+  gates.swap(window_);
+  last_plan_ = PlanInfo{};
+  if (planner_kind_ == 0 || (planner_kind_ == 1 && !model_)) {
+    // greedy max-5 blocks: the reference's rule, and the learned planner's fallback
+    std::vector<Block> blocks = greedy_blocks(gates);
+    last_plan_.kind = 0;
+    last_plan_.passes = (unsigned)blocks.size();
+    for (const Block &b : blocks) exec_block(b);
+    return;
+  }
+  const std::vector<PGate> pg = resolve(gates);
+  if (planner_kind_ == 2) {
+    unsigned t;
+    std::vector<TileProgram> progs = plan_tiles(pg, tile_tmax_, tile_rb_, tile_lmin_, &t);
+    last_plan_ = PlanInfo{1, tile_tmax_, tile_rb_, tile_lmin_, (unsigned)progs.size(), 1, 0.0};
+    for (const TileProgram &p : progs) exec_tile_program(p, tile_rb_);
+    return;
+  }
+  // learned planner: predict the time of every pass of every candidate plan, run the cheapest
+  struct Cand {
+    unsigned tmax, rb, lmin;
+  };
+  std::vector<Cand> cands;
+  if (c64_) cands = {{12, 4, 5}, {12, 4, 6}, {12, 4, 4}, {11, 4, 5}, {10, 4, 4}, {12, 5, 5}, {13, 5, 5}};
+  else cands = {{12, 4, 4}, {12, 4, 5}, {11, 4, 4}, {10, 4, 4}, {12, 5, 4}};
+  std::vector<std::vector<TileProgram>> plans;
+  std::vector<float> X;
+  std::vector<size_t> rows_of;
+  for (const Cand &c : cands) {
+    unsigned t;
+    plans.push_back(plan_tiles(pg, c.tmax, c.rb, c.lmin, &t));
+    for (const TileProgram &p : plans.back()) {
+      PassFeatures f = tile_pass_features(p, n_, c64_, c.rb);
+      X.insert(X.end(), f.f, f.f + kPassFeatures);
+    }
+    rows_of.push_back(plans.back().size());
+  }
+  std::vector<Block> blocks = greedy_blocks(gates);
+  for (const Block &b : blocks) {
+    PassFeatures f = block_features(b);
+    X.insert(X.end(), f.f, f.f + kPassFeatures);
+  }
+  rows_of.push_back(blocks.size());
+  const std::vector<float> y = model_->predict(X, X.size() / kPassFeatures, stream_);
+  size_t r = 0, best = 0;
+  double best_ms = 0.0;
+  for (size_t c = 0; c < rows_of.size(); ++c) {
+    double ms = 0.0;
+    for (size_t i = 0; i < rows_of[c]; ++i, ++r) ms += pass_time_from_target(y[r], n_);
+    if (c == 0 || ms < best_ms) {
+      best_ms = ms;
+      best = c;
+    }
+  }
+  last_plan_.candidates = (unsigned)rows_of.size();
+  last_plan_.predicted_ms = best_ms;
+  if (best == cands.size()) {
+    last_plan_.kind = 0;
+    last_plan_.passes = (unsigned)blocks.size();
+    for (const Block &b : blocks) exec_block(b);
+    return;
+  }
+  const Cand &c = cands[best];
+  last_plan_.kind = 1;
+  last_plan_.tmax = c.tmax;
+  last_plan_.rb = c.rb;
+  last_plan_.lmin = c.lmin;
+  last_plan_.passes = (unsigned)plans[best].size();
+  for (const TileProgram &p : plans[best]) exec_tile_program(p, c.rb);
+}
+
+void Engine::begin_pass(hipEvent_t *e0, hipEvent_t *e1) {
+  *e0 = *e1 = nullptr;
+  if (!profiling_ && !recording_) return;
+  *e0 = take_event();
+  *e1 = take_event();
+  QS_HIP(hipEventRecord(*e0, stream_));
+}
+
+void Engine::end_pass(hipEvent_t e0, hipEvent_t e1, const PassFeatures *f) {
+  if (!e0) return;
+  QS_HIP(hipEventRecord(e1, stream_));
+  if (recording_ && f) {
+    records_.push_back({*f, e0, e1});  // timed through take_records, not through the stats
+  } else {
+    ev_used_.push_back({e0, e1});
+  }
+}
+
+void Engine::exec_tile_program(const TileProgram &prog, unsigned rb) {
   if (prog.stages.empty()) return;
+  const unsigned t = prog.P.t;
   const uint64_t n_tiles = amps() >> t;
   auto up = [&](const auto &v) -> const void * {
     return v.empty() ? nullptr : upload(v.data(), v.size() * sizeof(v[0]));
@@ -415,6 +517,8 @@ void Engine::exec_tile_pass(const std::vector<PGate> &gates, const PPass &ps, un
   const dev::TileStage *d_st = (const dev::TileStage *)up(prog.stages);
   const dev::RegOp *d_rops = (const dev::RegOp *)up(prog.rops);
   const dev::TileOp *d_tops = (const dev::TileOp *)up(prog.tops);
+  PassFeatures f{};
+  if (recording_) f = tile_pass_features(prog, n_, c64_, rb);
   dispatch([&](auto tag) {
     using R = decltype(tag);
     std::vector<amp_t<R>> m(prog.mats.size());
@@ -423,17 +527,10 @@ void Engine::exec_tile_pass(const std::vector<PGate> &gates, const PPass &ps, un
       m[i].y = (R)prog.mats[i].imag();
     }
     const amp_t<R> *dm = (const amp_t<R> *)up(m);
-    hipEvent_t e0 = nullptr, e1 = nullptr;
-    if (profiling_) {
-      e0 = take_event();
-      e1 = take_event();
-      QS_HIP(hipEventRecord(e0, stream_));
-    }
+    hipEvent_t e0, e1;
+    begin_pass(&e0, &e1);
     dev::tile_pass<R>((amp_t<R> *)psi_, n_tiles, prog.P, (int)rb, d_st, d_rops, d_tops, dm, stream_);
-    if (profiling_) {
-      QS_HIP(hipEventRecord(e1, stream_));
-      ev_used_.push_back({e0, e1});
-    }
+    end_pass(e0, e1, recording_ ? &f : nullptr);
   });
   count_pass(2.0 * (double)amp_bytes() * (double)amps());
 }
@@ -462,20 +559,57 @@ void Engine::exec_block(const Block &b) {
       m[i].y = (R)b.m[i].imag();
     }
     const amp_t<R> *dm = (const amp_t<R> *)upload(m.data(), m.size() * sizeof(amp_t<R>));
-    hipEvent_t e0 = nullptr, e1 = nullptr;
-    if (profiling_) {
-      e0 = take_event();
-      e1 = take_event();
-      QS_HIP(hipEventRecord(e0, stream_));
-    }
+    PassFeatures f{};
+    if (recording_) f = block_features(b);
+    hipEvent_t e0, e1;
+    begin_pass(&e0, &e1);
     dev::apply_block<R>((amp_t<R> *)psi_, n_groups, ins, cmask, tpos.data(), k, dm, stream_);
-    if (profiling_) {
-      QS_HIP(hipEventRecord(e1, stream_));
-      ev_used_.push_back({e0, e1});
-    }
+    end_pass(e0, e1, recording_ ? &f : nullptr);
   });
   // algorithmic traffic: every touched amplitude is read once and written once
   count_pass(2.0 * (double)amp_bytes() * (double)(n_groups << k));
+}
+
+// ------------------------------------------------------------------------------ planner controls
+void Engine::set_tile_params(unsigned tmax, unsigned rb, unsigned lmin) {
+  if (tmax < 6 || tmax > (c64_ ? 13u : 12u) || (rb != 4 && rb != 5) || lmin < 1 || lmin > tmax)
+    throw ValueError("tile parameters out of range");
+  run();
+  tile_tmax_ = tmax;
+  tile_rb_ = rb;
+  tile_lmin_ = lmin;
+}
+
+void Engine::set_recording(bool on) {
+  run();
+  recording_ = on;
+}
+
+size_t Engine::take_records(float *features, float *ms, size_t cap) {
+  run();
+  sync();
+  size_t k = 0;
+  for (auto &r : records_) {
+    if (k < cap) {
+      float t = 0.f;
+      QS_HIP(hipEventElapsedTime(&t, r.a, r.b));
+      std::copy(r.f.f, r.f.f + kPassFeatures, features + k * kPassFeatures);
+      ms[k] = t;
+      ++k;
+    }
+    ev_free_.push_back(r.a);
+    ev_free_.push_back(r.b);
+  }
+  records_.clear();
+  return k;
+}
+
+void Engine::set_model(const std::vector<float> &w) {
+  if (w.empty()) {
+    model_.reset();
+    return;
+  }
+  model_ = std::make_unique<mlp::Model>(w);
 }
 
 // ------------------------------------------------------------------------------ measurement hooks

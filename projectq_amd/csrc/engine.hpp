@@ -8,6 +8,7 @@
 #include <cstdint>
 #include <functional>
 #include <map>
+#include <memory>
 #include <random>
 #include <utility>
 #include <vector>
@@ -15,6 +16,7 @@
 #include "common.hpp"
 #include "fusion.hpp"
 #include "kernels.hpp"
+#include "mlp.hpp"
 #include "planner.hpp"
 
 namespace qs {
@@ -61,6 +63,18 @@ class Engine {
   // execute one fused block (also used by the planners)
   void exec_block(const Block &b);
 
+  // planner controls (C-ABI extensions)
+  void set_tile_params(unsigned tmax, unsigned rb, unsigned lmin);
+  void set_recording(bool on);
+  size_t take_records(float *features, float *ms, size_t cap);
+  void set_model(const std::vector<float> &w);
+  struct PlanInfo {
+    int kind = -1;  // 0 greedy blocks, 1 tile passes
+    unsigned tmax = 0, rb = 0, lmin = 0, passes = 0, candidates = 0;
+    double predicted_ms = 0.0;
+  };
+  const PlanInfo &last_plan() const { return last_plan_; }
+
   // measurement hooks (C-ABI extensions)
   void synchronize();
   void set_profiling(bool on) { profiling_ = on; }
@@ -95,10 +109,25 @@ class Engine {
   void math_apply(dev::MathArgs args, const int32_t *d_table, bool known_bijective);
   double masked_norm(uint64_t mask, uint64_t val);
   void flush_window();
-  void flush_window_tiles(std::vector<Gate> &gates);
-  unsigned tile_bits() const;
   unsigned tile_tmax_ = 13, tile_rb_ = 5;  // tile shape limits (PROJECTQ_AMD_TILE="t:rb" overrides)
-  void exec_tile_pass(const std::vector<PGate> &gates, const PPass &pass, unsigned t);
+  unsigned tile_bits(unsigned tmax) const;
+  std::vector<Block> greedy_blocks(const std::vector<Gate> &gates) const;
+  std::vector<PGate> resolve(const std::vector<Gate> &gates) const;
+  std::vector<TileProgram> plan_tiles(const std::vector<PGate> &pg, unsigned tmax, unsigned rb, unsigned lmin,
+                                      unsigned *t_used) const;
+  void exec_tile_program(const TileProgram &prog, unsigned rb);
+  PassFeatures block_features(const Block &b) const;
+  void begin_pass(hipEvent_t *e0, hipEvent_t *e1);
+  void end_pass(hipEvent_t e0, hipEvent_t e1, const PassFeatures *f);
+  unsigned tile_lmin_ = 5;
+  bool recording_ = false;
+  struct PassRecord {
+    PassFeatures f;
+    hipEvent_t a, b;
+  };
+  std::vector<PassRecord> records_;
+  std::unique_ptr<mlp::Model> model_;
+  PlanInfo last_plan_;
 
   bool c64_;
   int fusion_mode_, planner_kind_;

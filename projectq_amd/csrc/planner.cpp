@@ -313,6 +313,60 @@ void build_tile_pass(const std::vector<PGate> &gates, const PPass &ps, unsigned 
   P.n_stages = (uint32_t)prog.stages.size();
 }
 
+PassFeatures tile_pass_features(const TileProgram &prog, unsigned n, bool c64, unsigned rb) {
+  PassFeatures F{};
+  float *f = F.f;
+  const dev::TilePass &P = prog.P;
+  f[0] = c64 ? 1.f : 0.f;
+  f[1] = n / 32.f;
+  f[2] = 0.f;
+  f[3] = P.t / 13.f;
+  f[4] = P.L / 13.f;
+  f[5] = P.h / 9.f;
+  f[6] = (P.h ? P.hpos[0] : n) / 32.f;
+  f[7] = rb / 5.f;
+  float reg = 0, lds = 0, kinds[7] = {0, 0, 0, 0, 0, 0, 0}, rctl = 0, lctl = 0, gctl = 0, ldsflops = 0;
+  for (const auto &st : prog.stages) (st.kind == dev::kStageRegisters ? reg : lds) += 1;
+  for (const auto &op : prog.rops) {
+    kinds[std::min<uint32_t>(op.code >> 8, 6)] += 1;
+    rctl += op.rcmask != 0;
+    lctl += op.lcmask != 0;
+    gctl += op.gcmask != 0;
+  }
+  for (const auto &op : prog.tops) {
+    ldsflops += float(1u << (2 * op.k));
+    lctl += op.lcmask != 0;
+    gctl += op.gcmask != 0;
+  }
+  f[8] = reg / 16.f;
+  f[9] = lds / 16.f;
+  f[10] = kinds[dev::kOp1General] / 32.f;
+  f[11] = kinds[dev::kOp1Real] / 32.f;
+  f[12] = kinds[dev::kOp1Diag] / 32.f;
+  f[13] = (kinds[dev::kOp1Anti] + kinds[dev::kOp1X]) / 32.f;
+  f[14] = kinds[dev::kOp2General] / 32.f;
+  f[15] = kinds[dev::kOp2Diag] / 32.f;
+  f[16] = rctl / 32.f;
+  f[17] = lctl / 32.f;
+  f[18] = gctl / 32.f;
+  f[19] = ldsflops / 1024.f;
+  f[23] = std::log2(1.f + float(prog.rops.size() + prog.tops.size())) / 8.f;
+  return F;
+}
+
+PassFeatures block_pass_features(unsigned k, unsigned n_ctrl, unsigned min_target, unsigned n, bool c64) {
+  PassFeatures F{};
+  float *f = F.f;
+  f[0] = c64 ? 1.f : 0.f;
+  f[1] = n / 32.f;
+  f[2] = 1.f;
+  f[20] = k / 5.f;
+  f[21] = n_ctrl / 8.f;
+  f[22] = min_target / 32.f;
+  f[23] = 1.f / 8.f;
+  return F;
+}
+
 void encode_pauli(const std::vector<std::pair<uint32_t, char>> &ops, uint64_t *x, uint64_t *z,
                   std::complex<double> *omega) {
   uint64_t X = 0, Z = 0;

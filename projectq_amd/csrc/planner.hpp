@@ -7,6 +7,7 @@
 // sequence up to rounding.
 #pragma once
 
+#include <cmath>
 #include <complex>
 #include <cstdint>
 #include <functional>
@@ -83,6 +84,19 @@ void build_tile_pass(const std::vector<PGate> &gates, const PPass &pass, unsigne
 // omega *= i (-1)^{x_q}, x ^= q, z ^= q.  Then (P psi)_i = omega (-1)^{popcount((i^x)&z)} psi_{i^x}.
 void encode_pauli(const std::vector<std::pair<uint32_t, char>> &ops, uint64_t *x, uint64_t *z,
                   std::complex<double> *omega);
+
+// ---- cost-model features of one pass (inputs of the learned planner's MLP) -----------------
+constexpr int kPassFeatures = 24;
+struct PassFeatures {
+  float f[kPassFeatures];
+};
+// a tile pass (program built with `rb` register bits) on an n-qubit state
+PassFeatures tile_pass_features(const TileProgram &prog, unsigned n, bool c64, unsigned rb);
+// a dense-block pass (k targets, c controls, lowest target position) on an n-qubit state
+PassFeatures block_pass_features(unsigned k, unsigned n_ctrl, unsigned min_target, unsigned n, bool c64);
+// the regression target: log of the pass time per 2^20 amplitudes
+inline double pass_time_target(double ms, unsigned n) { return std::log(ms) - (double(n) - 20.0) * std::log(2.0); }
+inline double pass_time_from_target(double y, unsigned n) { return std::exp(y + (double(n) - 20.0) * std::log(2.0)); }
 
 // Default slot choice: highest lookahead score first.
 std::vector<uint32_t> choose_by_score(const std::vector<SlotCandidate> &cands, unsigned slots);
