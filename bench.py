@@ -69,8 +69,9 @@ def gpu_headline(args, cmds, n):
     sim.synchronize()
     t1 = time.perf_counter()
     stats = sim.stats()
+    stats["plan"] = sim.last_plan()
     n_gates = sum(len(layer) for layer in timed)
-    log(f"headline: {len(timed)} layers, {n_gates} gates in {t1 - t0:.3f}s, {stats['passes']} passes")
+    log(f"headline: {len(timed)} layers, {n_gates} gates in {t1 - t0:.3f}s, {stats['passes']} passes, plan {stats['plan']}")
     del sim
     return t1 - t0, n_gates, stats
 
@@ -193,7 +194,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=2, help="untimed circuit layers before the timed ones")
     ap.add_argument("--qubits", type=int, default=32)
     ap.add_argument("--precision", default="c64", choices=["c64", "c128"])
-    ap.add_argument("--planner", default="tile", choices=["greedy", "tile", "learned"])
+    ap.add_argument("--planner", default="learned", choices=["greedy", "tile", "learned"])
     ap.add_argument("--plain-layers", type=int, default=2)
     ap.add_argument("--cpu-qubits", type=int, default=28)
     ap.add_argument("--cpu-layers", type=int, default=1)
@@ -237,6 +238,7 @@ def main():
         "passes": stats["passes"],
         "bytes_per_pass": stats["bytes"] / max(stats["passes"], 1),
         "avg_pass_ms": stats["gpu_ms"] / max(stats["timed_passes"], 1),
+        "plan": stats["plan"],
     }
 
     out = {

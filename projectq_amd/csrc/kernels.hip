@@ -392,16 +392,37 @@ __device__ __forceinline__ void reg_stage(amp_t<R> *__restrict__ tile, uint32_t 
 }
 
 // 16-byte vectors of amplitudes used to stream tiles (2 x complex64 or 1 x complex128)
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef double f64x2 __attribute__((ext_vector_type(2)));
 template <typename R>
 struct Vec16 {
-  using type = float4;
+  using type = f32x4;
   static constexpr int n = 2;
 };
 template <>
 struct Vec16<double> {
-  using type = double2;
+  using type = f64x2;
   static constexpr int n = 1;
 };
+
+// Streaming accesses of the tile passes: every amplitude is read and written exactly once per
+// pass, so the accesses carry the non-temporal hint unless QS_TILE_TEMPORAL is defined.
+template <typename V>
+__device__ __forceinline__ V stream_load(const V *p) {
+#ifdef QS_TILE_TEMPORAL
+  return *p;
+#else
+  return __builtin_nontemporal_load(p);
+#endif
+}
+template <typename V>
+__device__ __forceinline__ void stream_store(V *p, const V &v) {
+#ifdef QS_TILE_TEMPORAL
+  *p = v;
+#else
+  __builtin_nontemporal_store(v, p);
+#endif
+}
 
 // One tile of one pass: load the 2^t amplitudes into LDS, run the stages, store them back.
 // `seg` holds the 2^h segment offsets of the pass (shared memory, filled by the caller).
@@ -418,7 +439,7 @@ __device__ __forceinline__ void tile_body(amp_t<R> *__restrict__ psi, uint64_t t
   const uint64_t base = insert_zeros(tix, P.tiles);
   for (uint32_t l = threadIdx.x * VN; l < tsize; l += kBlock * VN) {
     const V *src = reinterpret_cast<const V *>(psi + (base | seg[l >> L] | (l & segmask)));
-    *reinterpret_cast<V *>(tile + slot<R>(l)) = *src;
+    *reinterpret_cast<V *>(tile + slot<R>(l)) = stream_load(src);
   }
   __syncthreads();
   for (uint32_t s = 0; s < P.n_stages; ++s) {
@@ -443,7 +464,7 @@ __device__ __forceinline__ void tile_body(amp_t<R> *__restrict__ psi, uint64_t t
   }
   for (uint32_t l = threadIdx.x * VN; l < tsize; l += kBlock * VN) {
     V *dst = reinterpret_cast<V *>(psi + (base | seg[l >> L] | (l & segmask)));
-    *dst = *reinterpret_cast<const V *>(tile + slot<R>(l));
+    stream_store(dst, *reinterpret_cast<const V *>(tile + slot<R>(l)));
   }
   __syncthreads();
 }

@@ -1,0 +1,74 @@
+"""The learned planner on the GPU: its answers equal the unfused complex128 run (the planner
+changes speed only), and what the trained network gains over the method without it -- the
+greedy max-5 planner it falls back to -- and against the heuristic tile planner."""
+
+import numpy as np
+import pytest
+
+from projectq_amd import circuits
+
+pytestmark = pytest.mark.gpu
+
+
+def execute(n, gates, **opts):
+    from projectq_amd import Simulator
+
+    sim = Simulator(11, capacity_qubits=n, **opts)
+    for q in range(n):
+        sim.allocate_qubit(q)
+    sim.run()
+    sim.synchronize()
+    sim.reset_stats()
+    sim.set_profiling(True)
+    window = opts.get("fusion") == "window"
+    for _, m, t, c in gates:
+        sim.apply_controlled_gate(m, list(t), list(c))
+        if not window:
+            sim.run()
+    sim.run()
+    sim.synchronize()
+    return sim, sim.stats()
+
+
+def infidelity(a, b):
+    return 1.0 - abs(a.overlap(b)) ** 2 / (a.overlap(a).real * b.overlap(b).real)
+
+
+def test_model_is_shipped():
+    from projectq_amd._cppsim import planner_model
+
+    w = planner_model()
+    assert w is not None and w.shape == (5825,) and np.all(np.isfinite(w))
+
+
+@pytest.mark.parametrize("n,depth", [(22, 40), (25, 30)])
+def test_learned_planner_gain_and_parity(n, depth):
+    gates = [c for c in circuits.mixed_circuit(n, depth, 900 + n) if c[0] == "gate"]
+    ref, _ = execute(n, gates, precision="c128", fusion="reference")
+    results = {}
+    for planner in ("greedy", "tile", "learned"):
+        sim, st = execute(n, gates, precision="c64", fusion="window", planner=planner)
+        inf = infidelity(ref, sim)
+        results[planner] = (st["gpu_ms"], st["passes"], inf, sim.last_plan())
+        del sim
+    print({k: (round(v[0], 3), v[1], f"{v[2]:.1e}", v[3]["kind"]) for k, v in results.items()}, flush=True)
+    for planner, (_, _, inf, _) in results.items():
+        assert inf <= 1e-6, planner
+    learned, greedy, tile = results["learned"][0], results["greedy"][0], results["tile"][0]
+    print(f"n={n}: learned gains {greedy / learned:.2f}x over greedy max-5, {tile / learned:.2f}x over the heuristic tile planner")
+    assert results["learned"][3]["candidates"] >= 2
+    assert learned < greedy
+    assert learned <= 1.25 * tile
+
+
+def test_learned_planner_c128_matches_oracle():
+    from oracle.cpu_sim import OracleSimulator
+    from projectq_amd import Simulator, commands
+
+    n = 14
+    cmds = circuits.mixed_circuit(n, 20, 55)
+    o = OracleSimulator(3)
+    commands.execute(o, cmds, False)
+    h = Simulator(3, precision="c128", fusion="window", planner="learned")
+    commands.execute(h, cmds, True)
+    assert np.max(np.abs(o.cheat_numpy()[1] - h.cheat_numpy()[1])) <= 1e-12
