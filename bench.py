@@ -187,6 +187,37 @@ def batched_variational(args):
     }
 
 
+def calibrate_copy(nbytes=1 << 31):
+    """One device-to-device copy of a known size, so hardware byte counters (FETCH_SIZE,
+    WRITE_SIZE) read over the same run can be scaled: the copy reads and writes nbytes."""
+    import ctypes
+
+    hip = ctypes.CDLL("libamdhip64.so")
+    a, b = ctypes.c_void_p(), ctypes.c_void_p()
+    if hip.hipMalloc(ctypes.byref(a), ctypes.c_size_t(nbytes)) or hip.hipMalloc(ctypes.byref(b), ctypes.c_size_t(nbytes)):
+        raise MemoryError("calibration buffers")
+    hip.hipMemset(a, 0, ctypes.c_size_t(nbytes))
+    hip.hipDeviceSynchronize()
+    hip.hipMemcpy(b, a, ctypes.c_size_t(nbytes), 3)  # hipMemcpyDeviceToDevice
+    hip.hipDeviceSynchronize()
+    hip.hipFree(a)
+    hip.hipFree(b)
+    log(f"calibration copy: {nbytes} bytes read and written")
+
+
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "k_tile_pass_random32_c64_traffic.json")
+
+
+def measured_traffic(args, n):
+    """HBM bytes per tile-pass launch from the committed FETCH_SIZE/WRITE_SIZE counter runs
+    (scaled by a 2 GiB copy measured in the same runs); only for the workload they measured."""
+    if n != 32 or args.precision != "c64" or args.planner == "greedy" or not os.path.exists(TRAFFIC_PROFILE):
+        return None, None
+    with open(TRAFFIC_PROFILE) as f:
+        rec = json.load(f)
+    return int(round(rec["traffic_bytes_per_launch"])), os.path.relpath(TRAFFIC_PROFILE, ROOT)
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--gpus", type=int, default=1)
@@ -203,6 +234,7 @@ def main():
     ap.add_argument("--seed", type=int, default=2024)
     ap.add_argument("--batch", type=int, default=512, help="circuits in the batched-variational extra")
     ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--calibrate-copy", action="store_true", help="add a known-size D2D copy (counter scaling)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -221,9 +253,12 @@ def main():
     log(f"circuit: {n} qubits, {total_layers} layers, {CNOTS_PER_LAYER} CNOTs/layer")
 
     wall, n_gates, stats = gpu_headline(args, cmds, n)
+    if args.calibrate_copy:
+        calibrate_copy()
     value = n_gates / wall
     ms_per_step = 1e3 * wall / args.steps
 
+    traffic, traffic_src = measured_traffic(args, n)
     achieved = None
     if stats["timed_passes"] and stats["gpu_ms"] > 0:
         achieved = stats["bytes"] / (stats["gpu_ms"] * 1e-3) / 1e9
@@ -233,7 +268,8 @@ def main():
         "peak": HBM_PEAK_GBS,
         "unit": "GB/s",
         "frac": round(achieved / HBM_PEAK_GBS, 4) if achieved else None,
-        "traffic": None,
+        "traffic": traffic,
+        "traffic_source": traffic_src,
         "kernel": "k_tile_pass (LDS tile pass)" if args.planner != "greedy" else "k_apply (dense fused-block gate pass)",
         "passes": stats["passes"],
         "bytes_per_pass": stats["bytes"] / max(stats["passes"], 1),
