@@ -187,6 +187,16 @@ def batched_variational(args):
     }
 
 
+def kernel_name(args, plan):
+    """The gate-pass kernel the plan ran: the tile launcher uses the software-pipelined variant for
+    complex64 tiles of 2^12 amplitudes with 4 register bits."""
+    if not plan or plan.get("kind") != "tile":
+        return "k_apply (dense fused-block gate pass)"
+    if args.precision == "c64" and plan.get("tile_bits") == 12 and plan.get("reg_bits") == 4:
+        return "k_tile_pass_pf (software-pipelined LDS tile pass)"
+    return "k_tile_pass (LDS tile pass)"
+
+
 def calibrate_copy(nbytes=1 << 31):
     """One device-to-device copy of a known size, so hardware byte counters (FETCH_SIZE,
     WRITE_SIZE) read over the same run can be scaled: the copy reads and writes nbytes."""
@@ -270,7 +280,7 @@ def main():
         "frac": round(achieved / HBM_PEAK_GBS, 4) if achieved else None,
         "traffic": traffic,
         "traffic_source": traffic_src,
-        "kernel": "k_tile_pass (LDS tile pass)" if args.planner != "greedy" else "k_apply (dense fused-block gate pass)",
+        "kernel": kernel_name(args, stats["plan"]),
         "passes": stats["passes"],
         "bytes_per_pass": stats["bytes"] / max(stats["passes"], 1),
         "avg_pass_ms": stats["gpu_ms"] / max(stats["timed_passes"], 1),

@@ -122,7 +122,7 @@ void Batch::build(const BatchShape &shape) {
     std::vector<PPass> passes = schedule_tiles(gates, ts);
     for (size_t p = 0; p < passes.size(); ++p) {
       TileProgram prog;
-      build_tile_pass(gates, passes[p], t, shape.rb, prog);
+      build_tile_pass(gates, passes[p], t, shape.rb, c64_, prog);
       if (prog.stages.empty()) continue;
       dev::BatchJob job{};
       job.state_off = state_off[c];

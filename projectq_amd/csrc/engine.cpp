@@ -403,7 +403,7 @@ std::vector<TileProgram> Engine::plan_tiles(const std::vector<PGate> &pg, unsign
   *t_used = t;
   std::vector<PPass> passes = schedule_tiles(pg, TileShape{n_, t, lmin});
   std::vector<TileProgram> progs(passes.size());
-  for (size_t i = 0; i < passes.size(); ++i) build_tile_pass(pg, passes[i], t, rb, progs[i]);
+  for (size_t i = 0; i < passes.size(); ++i) build_tile_pass(pg, passes[i], t, rb, c64_, progs[i]);
   return progs;
 }
 

@@ -145,6 +145,48 @@ template <>
 __device__ __forceinline__ uint32_t slot<double>(uint32_t x) {
   return lds_slot_c128(x);
 }
+// Both swizzles are linear over GF(2): for indices a, b without common bits,
+// slot(a | b) = slot(a) ^ slot(b).  The kernels use this to address a group of amplitudes with
+// one XOR per access: the per-thread part slot(lb) is computed once and the per-amplitude parts
+// slot(offset) are uniform over the workgroup.  slot_bytes gives the byte offset in the tile.
+template <typename R>
+__device__ __forceinline__ uint32_t slot_bytes(uint32_t x) {
+  return slot<R>(x) * (uint32_t)sizeof(amp_t<R>);
+}
+// Tiles are addressed by raw LDS byte address.  The tile kernels declare no static LDS, so the
+// dynamic buffer (and the tile) starts at LDS address 0 and base ^ offset == base + offset; the
+// XOR-composed offsets then need no add before the ds instruction.
+template <typename T>
+using lds_ptr = __attribute__((address_space(3))) T *;
+__device__ __forceinline__ uint32_t lds_address(const void *p) {
+  return (uint32_t)(uintptr_t)(lds_ptr<const char>)(p);
+}
+// T is a builtin vector type (16-byte tile streaming); amplitudes go through lds_*_amp.
+template <typename T>
+__device__ __forceinline__ T lds_load(uint32_t lds_byte_addr) {
+  return *(lds_ptr<const T>)(lds_byte_addr);
+}
+template <typename T>
+__device__ __forceinline__ void lds_store(uint32_t lds_byte_addr, const T &v) {
+  *(lds_ptr<T>)(lds_byte_addr) = v;
+}
+template <typename R>
+using re2_t = R __attribute__((ext_vector_type(2)));
+template <typename R>
+__device__ __forceinline__ amp_t<R> lds_load_amp(uint32_t lds_byte_addr) {
+  const re2_t<R> v = lds_load<re2_t<R>>(lds_byte_addr);
+  amp_t<R> a;
+  a.x = v.x;
+  a.y = v.y;
+  return a;
+}
+template <typename R>
+__device__ __forceinline__ void lds_store_amp(uint32_t lds_byte_addr, const amp_t<R> &a) {
+  re2_t<R> v;
+  v.x = a.x;
+  v.y = a.y;
+  lds_store<re2_t<R>>(lds_byte_addr, v);
+}
 
 // One TileOp of any width straight on the LDS tile.
 template <typename R, int K>
@@ -161,13 +203,16 @@ __device__ __forceinline__ void tile_apply(amp_t<R> *__restrict__ tile, uint32_t
       lb = ((lb >> p) << (p + 1)) | (lb & ((1u << p) - 1));
     }
     lb |= lcmask;
+    uint32_t sb = lds_address(tile) + slot_bytes<R>(lb);
+    asm volatile("" : "+v"(sb));
     R vr[D], vi[D];
 #pragma unroll
     for (int j = 0; j < D; ++j) {
-      amp_t<R> a = tile[slot<R>(lb + op->off[j])];
+      const amp_t<R> a = lds_load_amp<R>(sb ^ slot_bytes<R>(op->off[j]));
       vr[j] = a.x;
       vi[j] = a.y;
     }
+    asm volatile("" : "+v"(sb));
 #pragma unroll(K <= 3 ? D : 1)
     for (int r = 0; r < D; ++r) {
       R ar = 0, ai = 0;
@@ -182,21 +227,24 @@ __device__ __forceinline__ void tile_apply(amp_t<R> *__restrict__ tile, uint32_t
       amp_t<R> o;
       o.x = ar;
       o.y = ai;
-      tile[slot<R>(lb + op->off[r])] = o;
+      lds_store_amp<R>(sb ^ slot_bytes<R>(op->off[r]), o);
     }
   }
 }
 
 // ---- register-stage ops: NR = 2^RB amplitudes of one thread, register index bit i <-> stage
-// bit i; rc = controls among register bits (a compile-time index i passes iff (i & rc) == rc).
-template <typename R, int RB, bool RC, int B>
-__device__ __forceinline__ void rop_general(R *vr, R *vi, uint32_t rc, const amp_t<R> *__restrict__ m) {
+// bit i.  C is the op's control among the register bits (-1: none); it is a compile-time
+// constant, so a controlled op touches only the indices with bit C set and a controlled X is a
+// renaming of registers.
+__host__ __device__ constexpr bool reg_ctrl_ok(int i, int C) { return C < 0 || ((i >> (C < 0 ? 0 : C)) & 1); }
+
+template <typename R, int RB, int C, int B>
+__device__ __forceinline__ void rop_general(R *vr, R *vi, const amp_t<R> *__restrict__ m) {
   const R ar = m[0].x, ai = m[0].y, br = m[1].x, bi = m[1].y;
   const R cr = m[2].x, ci = m[2].y, dr = m[3].x, di = m[3].y;
 #pragma unroll
   for (int i = 0; i < (1 << RB); ++i) {
-    if (i & (1 << B)) continue;
-    if (RC && (i & rc) != rc) continue;
+    if ((i & (1 << B)) || !reg_ctrl_ok(i, C)) continue;
     const int j = i | (1 << B);
     const R xr = vr[i], xi = vi[i], yr = vr[j], yi = vi[j];
     vr[i] = fma(ar, xr, fma(-ai, xi, fma(br, yr, -bi * yi)));
@@ -205,13 +253,12 @@ __device__ __forceinline__ void rop_general(R *vr, R *vi, uint32_t rc, const amp
     vi[j] = fma(cr, xi, fma(ci, xr, fma(dr, yi, di * yr)));
   }
 }
-template <typename R, int RB, bool RC, int B>
-__device__ __forceinline__ void rop_real(R *vr, R *vi, uint32_t rc, const amp_t<R> *__restrict__ m) {
+template <typename R, int RB, int C, int B>
+__device__ __forceinline__ void rop_real(R *vr, R *vi, const amp_t<R> *__restrict__ m) {
   const R a = m[0].x, b = m[1].x, c = m[2].x, d = m[3].x;
 #pragma unroll
   for (int i = 0; i < (1 << RB); ++i) {
-    if (i & (1 << B)) continue;
-    if (RC && (i & rc) != rc) continue;
+    if ((i & (1 << B)) || !reg_ctrl_ok(i, C)) continue;
     const int j = i | (1 << B);
     const R xr = vr[i], xi = vi[i], yr = vr[j], yi = vi[j];
     vr[i] = fma(a, xr, b * yr);
@@ -220,25 +267,24 @@ __device__ __forceinline__ void rop_real(R *vr, R *vi, uint32_t rc, const amp_t<
     vi[j] = fma(c, xi, d * yi);
   }
 }
-template <typename R, int RB, bool RC, int B>
-__device__ __forceinline__ void rop_diag(R *vr, R *vi, uint32_t rc, const amp_t<R> *__restrict__ m) {
+template <typename R, int RB, int C, int B>
+__device__ __forceinline__ void rop_diag(R *vr, R *vi, const amp_t<R> *__restrict__ m) {
   const R ar = m[0].x, ai = m[0].y, dr = m[3].x, di = m[3].y;
 #pragma unroll
   for (int i = 0; i < (1 << RB); ++i) {
-    if (RC && (i & rc) != rc) continue;
+    if (!reg_ctrl_ok(i, C)) continue;
     const R fr = (i & (1 << B)) ? dr : ar, fi = (i & (1 << B)) ? di : ai;
     const R xr = vr[i], xi = vi[i];
     vr[i] = fma(fr, xr, -fi * xi);
     vi[i] = fma(fr, xi, fi * xr);
   }
 }
-template <typename R, int RB, bool RC, int B>
-__device__ __forceinline__ void rop_anti(R *vr, R *vi, uint32_t rc, const amp_t<R> *__restrict__ m) {
+template <typename R, int RB, int C, int B>
+__device__ __forceinline__ void rop_anti(R *vr, R *vi, const amp_t<R> *__restrict__ m) {
   const R br = m[1].x, bi = m[1].y, cr = m[2].x, ci = m[2].y;
 #pragma unroll
   for (int i = 0; i < (1 << RB); ++i) {
-    if (i & (1 << B)) continue;
-    if (RC && (i & rc) != rc) continue;
+    if ((i & (1 << B)) || !reg_ctrl_ok(i, C)) continue;
     const int j = i | (1 << B);
     const R xr = vr[i], xi = vi[i], yr = vr[j], yi = vi[j];
     vr[i] = fma(br, yr, -bi * yi);
@@ -247,12 +293,11 @@ __device__ __forceinline__ void rop_anti(R *vr, R *vi, uint32_t rc, const amp_t<
     vi[j] = fma(cr, xi, ci * xr);
   }
 }
-template <typename R, int RB, bool RC, int B>
-__device__ __forceinline__ void rop_x(R *vr, R *vi, uint32_t rc, const amp_t<R> *__restrict__) {
+template <typename R, int RB, int C, int B>
+__device__ __forceinline__ void rop_x(R *vr, R *vi, const amp_t<R> *__restrict__) {
 #pragma unroll
   for (int i = 0; i < (1 << RB); ++i) {
-    if (i & (1 << B)) continue;
-    if (RC && (i & rc) != rc) continue;
+    if ((i & (1 << B)) || !reg_ctrl_ok(i, C)) continue;
     const int j = i | (1 << B);
     const R xr = vr[i], xi = vi[i];
     vr[i] = vr[j];
@@ -261,12 +306,11 @@ __device__ __forceinline__ void rop_x(R *vr, R *vi, uint32_t rc, const amp_t<R> 
     vi[j] = xi;
   }
 }
-template <typename R, int RB, bool RC, int B0, int B1>
-__device__ __forceinline__ void rop_general2(R *vr, R *vi, uint32_t rc, const amp_t<R> *__restrict__ m) {
+template <typename R, int RB, int C, int B0, int B1>
+__device__ __forceinline__ void rop_general2(R *vr, R *vi, const amp_t<R> *__restrict__ m) {
 #pragma unroll
   for (int i = 0; i < (1 << RB); ++i) {
-    if (i & ((1 << B0) | (1 << B1))) continue;
-    if (RC && (i & rc) != rc) continue;
+    if ((i & ((1 << B0) | (1 << B1))) || !reg_ctrl_ok(i, C)) continue;
     const int idx[4] = {i, i | (1 << B0), i | (1 << B1), i | (1 << B0) | (1 << B1)};
     R xr[4], xi[4];
 #pragma unroll
@@ -288,11 +332,11 @@ __device__ __forceinline__ void rop_general2(R *vr, R *vi, uint32_t rc, const am
     }
   }
 }
-template <typename R, int RB, bool RC, int B0, int B1>
-__device__ __forceinline__ void rop_diag2(R *vr, R *vi, uint32_t rc, const amp_t<R> *__restrict__ m) {
+template <typename R, int RB, int C, int B0, int B1>
+__device__ __forceinline__ void rop_diag2(R *vr, R *vi, const amp_t<R> *__restrict__ m) {
 #pragma unroll
   for (int i = 0; i < (1 << RB); ++i) {
-    if (RC && (i & rc) != rc) continue;
+    if (!reg_ctrl_ok(i, C)) continue;
     const int s = ((i >> B0) & 1) | (((i >> B1) & 1) << 1);
     const amp_t<R> f = m[s * 5];  // diagonal element s of the 4x4 matrix
     const R xr = vr[i], xi = vi[i];
@@ -301,44 +345,72 @@ __device__ __forceinline__ void rop_diag2(R *vr, R *vi, uint32_t rc, const amp_t
   }
 }
 
-#define QS_ROP1(FN)                                                                     \
-  switch (b0) {                                                                         \
-    case 0: FN<R, RB, RC, 0>(vr, vi, rc, m); break;                                         \
-    case 1: FN<R, RB, RC, 1>(vr, vi, rc, m); break;                                         \
-    case 2: FN<R, RB, RC, 2>(vr, vi, rc, m); break;                                         \
-    case 3: FN<R, RB, RC, 3>(vr, vi, rc, m); break;                                         \
-    default:                                                                            \
-      if constexpr (RB > 4) FN<R, RB, RC, 4>(vr, vi, rc, m);                                \
-      break;                                                                            \
+// Dispatch on the kind, register bits and register control of an op (cb = 1 + control bit, 0 =
+// none), in one function so that the amplitude registers stay in registers.  Combinations the
+// planner never emits (a control on a target, bits >= RB) do nothing.
+#define QS_R1(FN, C, B)                                          \
+  if constexpr ((B) < RB && (C) < RB && (C) != (B)) FN<R, RB, C, B>(vr, vi, m);
+#define QS_R1C(FN, B)                                            \
+  switch (cb) {                                                  \
+    case 0: QS_R1(FN, -1, B) break;                              \
+    case 1: QS_R1(FN, 0, B) break;                               \
+    case 2: QS_R1(FN, 1, B) break;                               \
+    case 3: QS_R1(FN, 2, B) break;                               \
+    case 4: QS_R1(FN, 3, B) break;                               \
+    default: QS_R1(FN, 4, B) break;                              \
   }
-#define QS_ROP2(FN)                                                                     \
-  switch (b0 * 8 + b1) {                                                                \
-    case 0 * 8 + 1: FN<R, RB, RC, 0, 1>(vr, vi, rc, m); break;                              \
-    case 0 * 8 + 2: FN<R, RB, RC, 0, 2>(vr, vi, rc, m); break;                              \
-    case 0 * 8 + 3: FN<R, RB, RC, 0, 3>(vr, vi, rc, m); break;                              \
-    case 1 * 8 + 2: FN<R, RB, RC, 1, 2>(vr, vi, rc, m); break;                              \
-    case 1 * 8 + 3: FN<R, RB, RC, 1, 3>(vr, vi, rc, m); break;                              \
-    case 2 * 8 + 3: FN<R, RB, RC, 2, 3>(vr, vi, rc, m); break;                              \
-    case 0 * 8 + 4: if constexpr (RB > 4) FN<R, RB, RC, 0, 4>(vr, vi, rc, m); break;        \
-    case 1 * 8 + 4: if constexpr (RB > 4) FN<R, RB, RC, 1, 4>(vr, vi, rc, m); break;        \
-    case 2 * 8 + 4: if constexpr (RB > 4) FN<R, RB, RC, 2, 4>(vr, vi, rc, m); break;        \
-    default: if constexpr (RB > 4) FN<R, RB, RC, 3, 4>(vr, vi, rc, m); break;               \
+#define QS_R1B(FN)                                               \
+  switch (b0) {                                                  \
+    case 0: QS_R1C(FN, 0) break;                                 \
+    case 1: QS_R1C(FN, 1) break;                                 \
+    case 2: QS_R1C(FN, 2) break;                                 \
+    case 3: QS_R1C(FN, 3) break;                                 \
+    default: QS_R1C(FN, 4) break;                                \
+  }
+#define QS_R2(FN, C, B0, B1)                                                            \
+  if constexpr ((B1) < RB && (C) < RB && (C) != (B0) && (C) != (B1)) FN<R, RB, C, B0, B1>(vr, vi, m);
+#define QS_R2C(FN, B0, B1)                                       \
+  switch (cb) {                                                  \
+    case 0: QS_R2(FN, -1, B0, B1) break;                         \
+    case 1: QS_R2(FN, 0, B0, B1) break;                          \
+    case 2: QS_R2(FN, 1, B0, B1) break;                          \
+    case 3: QS_R2(FN, 2, B0, B1) break;                          \
+    case 4: QS_R2(FN, 3, B0, B1) break;                          \
+    default: QS_R2(FN, 4, B0, B1) break;                         \
+  }
+#define QS_R2B(FN)                                               \
+  switch (b0 * 8 + b1) {                                         \
+    case 0 * 8 + 1: QS_R2C(FN, 0, 1) break;                      \
+    case 0 * 8 + 2: QS_R2C(FN, 0, 2) break;                      \
+    case 0 * 8 + 3: QS_R2C(FN, 0, 3) break;                      \
+    case 1 * 8 + 2: QS_R2C(FN, 1, 2) break;                      \
+    case 1 * 8 + 3: QS_R2C(FN, 1, 3) break;                      \
+    case 2 * 8 + 3: QS_R2C(FN, 2, 3) break;                      \
+    case 0 * 8 + 4: QS_R2C(FN, 0, 4) break;                      \
+    case 1 * 8 + 4: QS_R2C(FN, 1, 4) break;                      \
+    case 2 * 8 + 4: QS_R2C(FN, 2, 4) break;                      \
+    default: QS_R2C(FN, 3, 4) break;                             \
   }
 
-template <typename R, int RB, bool RC>
-__device__ __forceinline__ void reg_op(uint32_t code, R *vr, R *vi, uint32_t rc,
-                                       const amp_t<R> *__restrict__ m) {
-  const uint32_t kind = code >> 8, b0 = code & 15u, b1 = (code >> 4) & 15u;
+template <typename R, int RB>
+__device__ __forceinline__ void reg_op(uint32_t code, R *vr, R *vi, const amp_t<R> *__restrict__ m) {
+  const uint32_t kind = (code >> 8) & 15u, b0 = code & 15u, b1 = (code >> 4) & 15u, cb = code >> 12;
   switch (kind) {
-    case kOp1General: QS_ROP1(rop_general) break;
-    case kOp1Real: QS_ROP1(rop_real) break;
-    case kOp1Diag: QS_ROP1(rop_diag) break;
-    case kOp1Anti: QS_ROP1(rop_anti) break;
-    case kOp1X: QS_ROP1(rop_x) break;
-    case kOp2General: QS_ROP2(rop_general2) break;
-    default: QS_ROP2(rop_diag2) break;
+    case kOp1General: QS_R1B(rop_general) break;
+    case kOp1Real: QS_R1B(rop_real) break;
+    case kOp1Diag: QS_R1B(rop_diag) break;
+    case kOp1Anti: QS_R1B(rop_anti) break;
+    case kOp1X: QS_R1B(rop_x) break;
+    case kOp2General: QS_R2B(rop_general2) break;
+    default: QS_R2B(rop_diag2) break;
   }
 }
+#undef QS_R1
+#undef QS_R1C
+#undef QS_R1B
+#undef QS_R2
+#undef QS_R2C
+#undef QS_R2B
 
 template <typename R, int RB>
 __device__ __forceinline__ void reg_stage(amp_t<R> *__restrict__ tile, uint32_t t, uint64_t base,
@@ -346,15 +418,9 @@ __device__ __forceinline__ void reg_stage(amp_t<R> *__restrict__ tile, uint32_t 
                                           const RegOp *__restrict__ rops,
                                           const amp_t<R> *__restrict__ mats) {
   constexpr int NR = 1 << RB;
-  uint32_t roff[NR];
+  uint32_t soff[NR];  // byte offset (swizzled) of register i relative to the group's base
 #pragma unroll
-  for (int i = 0; i < NR; ++i) {
-    uint32_t o = 0;
-#pragma unroll
-    for (int b = 0; b < RB; ++b)
-      if ((i >> b) & 1) o |= 1u << st->fixed[b];
-    roff[i] = o;
-  }
+  for (int i = 0; i < NR; ++i) soff[i] = st->soff[i];
   const uint32_t ngroups = 1u << (t - RB);
   const uint32_t ob = st->begin, oe = st->end;
   for (uint32_t g = threadIdx.x; g < ngroups; g += kBlock) {
@@ -364,10 +430,12 @@ __device__ __forceinline__ void reg_stage(amp_t<R> *__restrict__ tile, uint32_t 
       const uint32_t p = st->fixed[b];
       lb = ((lb >> p) << (p + 1)) | (lb & ((1u << p) - 1));
     }
+    uint32_t sb = lds_address(tile) + slot_bytes<R>(lb);
+    asm volatile("" : "+v"(sb));  // keeps the address math at one XOR per access
     R vr[NR], vi[NR];
 #pragma unroll
     for (int i = 0; i < NR; ++i) {
-      const amp_t<R> a = tile[slot<R>(lb | roff[i])];
+      const amp_t<R> a = lds_load_amp<R>(sb ^ soff[i]);
       vr[i] = a.x;
       vi[i] = a.y;
     }
@@ -377,16 +445,17 @@ __device__ __forceinline__ void reg_stage(amp_t<R> *__restrict__ tile, uint32_t 
       if ((base & gc) != gc) continue;
       const uint32_t lc = op->lcmask;
       if ((lb & lc) != lc) continue;
-      const uint32_t rc = op->rcmask;
-      if (rc) reg_op<R, RB, true>(op->code, vr, vi, rc, mats + op->mat);
-      else reg_op<R, RB, false>(op->code, vr, vi, 0u, mats + op->mat);
+      reg_op<R, RB>(op->code, vr, vi, mats + op->mat);
     }
+    // Recompute the LDS addresses for the write-back instead of keeping 2^RB of them live across
+    // the ops (the empty asm makes sb opaque to common-subexpression elimination).
+    asm volatile("" : "+v"(sb));
 #pragma unroll
     for (int i = 0; i < NR; ++i) {
       amp_t<R> a;
       a.x = vr[i];
       a.y = vi[i];
-      tile[slot<R>(lb | roff[i])] = a;
+      lds_store_amp<R>(sb ^ soff[i], a);
     }
   }
 }
@@ -404,6 +473,38 @@ struct Vec16<double> {
   using type = f64x2;
   static constexpr int n = 1;
 };
+
+// The LDS side of tile streaming: one 16-byte global vector holds one complex128 amplitude or
+// the complex64 pair (l, l + 1).  The swizzle keeps such a pair adjacent but may swap it
+// (slot(l + 1) = slot(l) ^ 1), so complex64 pairs move as two 8-byte accesses.
+template <typename R>
+__device__ __forceinline__ void tile_put16(uint32_t addr, const typename Vec16<R>::type &v) {
+  if constexpr (sizeof(R) == 4) {
+    re2_t<float> a, b;
+    a.x = v.x;
+    a.y = v.y;
+    b.x = v.z;
+    b.y = v.w;
+    lds_store<re2_t<float>>(addr, a);
+    lds_store<re2_t<float>>(addr ^ 8u, b);
+  } else {
+    lds_store<typename Vec16<R>::type>(addr, v);
+  }
+}
+template <typename R>
+__device__ __forceinline__ typename Vec16<R>::type tile_get16(uint32_t addr) {
+  if constexpr (sizeof(R) == 4) {
+    const re2_t<float> a = lds_load<re2_t<float>>(addr), b = lds_load<re2_t<float>>(addr ^ 8u);
+    typename Vec16<R>::type v;
+    v.x = a.x;
+    v.y = a.y;
+    v.z = b.x;
+    v.w = b.y;
+    return v;
+  } else {
+    return lds_load<typename Vec16<R>::type>(addr);
+  }
+}
 
 // Streaming accesses of the tile passes: every amplitude is read and written exactly once per
 // pass, so the accesses carry the non-temporal hint unless QS_TILE_TEMPORAL is defined.
@@ -424,25 +525,26 @@ __device__ __forceinline__ void stream_store(V *p, const V &v) {
 #endif
 }
 
-// One tile of one pass: load the 2^t amplitudes into LDS, run the stages, store them back.
-// `seg` holds the 2^h segment offsets of the pass (shared memory, filled by the caller).
+// Global index of the first amplitude of tile `tix`: the low L bits and the high tile positions
+// (ascending, all >= L) are zero and the remaining bits hold tix.  The loop has a fixed trip count
+// so the positions are read at static offsets of the kernel argument (scalar loads).
+__device__ __forceinline__ uint64_t tile_base(uint64_t tix, const TilePass &P) {
+  uint64_t g = tix << P.L;
+#pragma unroll
+  for (int b = 0; b < kTileMaxHigh; ++b)
+    if (b < (int)P.h) g = insert_one_zero(g, (int)P.hpos[b]);
+  return g;
+}
+
+// 16-byte accesses each thread keeps in flight while streaming a tile.
+constexpr int kStreamChunk = 8;
+
+// The stages of a pass on one tile resident in LDS, a workgroup barrier after each.
 template <typename R, int RB>
-__device__ __forceinline__ void tile_body(amp_t<R> *__restrict__ psi, uint64_t tix, const TilePass &P,
-                                          const uint64_t *seg, amp_t<R> *__restrict__ tile,
-                                          const TileStage *__restrict__ stages,
-                                          const RegOp *__restrict__ rops, const TileOp *__restrict__ tops,
-                                          const amp_t<R> *__restrict__ mats) {
-  using V = typename Vec16<R>::type;
-  constexpr int VN = Vec16<R>::n;
-  const uint32_t t = P.t, L = P.L;
-  const uint32_t tsize = 1u << t, segmask = (1u << L) - 1;
-  const uint64_t base = insert_zeros(tix, P.tiles);
-  for (uint32_t l = threadIdx.x * VN; l < tsize; l += kBlock * VN) {
-    const V *src = reinterpret_cast<const V *>(psi + (base | seg[l >> L] | (l & segmask)));
-    *reinterpret_cast<V *>(tile + slot<R>(l)) = stream_load(src);
-  }
-  __syncthreads();
-  for (uint32_t s = 0; s < P.n_stages; ++s) {
+__device__ __forceinline__ void run_stages(amp_t<R> *__restrict__ tile, uint32_t t, uint64_t base, uint32_t n_stages,
+                                           const TileStage *__restrict__ stages, const RegOp *__restrict__ rops,
+                                           const TileOp *__restrict__ tops, const amp_t<R> *__restrict__ mats) {
+  for (uint32_t s = 0; s < n_stages; ++s) {
     const TileStage *st = stages + s;
     if (st->kind == kStageRegisters) {
       reg_stage<R, RB>(tile, t, base, st, rops, mats);
@@ -462,9 +564,53 @@ __device__ __forceinline__ void tile_body(amp_t<R> *__restrict__ psi, uint64_t t
     }
     __syncthreads();
   }
-  for (uint32_t l = threadIdx.x * VN; l < tsize; l += kBlock * VN) {
-    V *dst = reinterpret_cast<V *>(psi + (base | seg[l >> L] | (l & segmask)));
-    stream_store(dst, *reinterpret_cast<const V *>(tile + slot<R>(l)));
+}
+
+// One tile of one pass: load the 2^t amplitudes into LDS, run the stages, store them back.
+// `seg` holds the 2^h segment offsets of the pass (shared memory, filled by the caller).
+// FULL: 2^t is a multiple of kStreamChunk * kBlock * VN, so every streamed vector exists and the
+// accesses need no guard (straight-line code lets the memory counters be waited on exactly).
+template <typename R, int RB, bool FULL>
+__device__ __forceinline__ void tile_body(amp_t<R> *__restrict__ psi, uint64_t tix, const TilePass &P,
+                                          const uint64_t *seg, amp_t<R> *__restrict__ tile,
+                                          const TileStage *__restrict__ stages,
+                                          const RegOp *__restrict__ rops, const TileOp *__restrict__ tops,
+                                          const amp_t<R> *__restrict__ mats) {
+  using V = typename Vec16<R>::type;
+  constexpr int VN = Vec16<R>::n;
+  constexpr uint32_t kStep = kBlock * VN;
+  const uint32_t t = P.t, L = P.L;
+  const uint32_t tsize = 1u << t, segmask = (1u << L) - 1;
+  const uint64_t base = tile_base(tix, P);
+  for (uint32_t l0 = threadIdx.x * VN; l0 < tsize; l0 += kStreamChunk * kStep) {
+    const uint32_t s0 = lds_address(tile) + slot_bytes<R>(l0);  // l0 shares no bit with j * kStep
+    V v[kStreamChunk];
+#pragma unroll
+    for (int j = 0; j < kStreamChunk; ++j) {
+      const uint32_t l = l0 + j * kStep;
+      if (FULL || l < tsize) v[j] = stream_load(reinterpret_cast<const V *>(psi + (base | seg[l >> L] | (l & segmask))));
+    }
+#pragma unroll
+    for (int j = 0; j < kStreamChunk; ++j) {
+      const uint32_t l = l0 + j * kStep;
+      if (FULL || l < tsize) tile_put16<R>(s0 ^ slot_bytes<R>(j * kStep), v[j]);
+    }
+  }
+  __syncthreads();
+  run_stages<R, RB>(tile, t, base, P.n_stages, stages, rops, tops, mats);
+  for (uint32_t l0 = threadIdx.x * VN; l0 < tsize; l0 += kStreamChunk * kStep) {
+    const uint32_t s0 = lds_address(tile) + slot_bytes<R>(l0);  // l0 shares no bit with j * kStep
+    V v[kStreamChunk];
+#pragma unroll
+    for (int j = 0; j < kStreamChunk; ++j) {
+      const uint32_t l = l0 + j * kStep;
+      if (FULL || l < tsize) v[j] = tile_get16<R>(s0 ^ slot_bytes<R>(j * kStep));
+    }
+#pragma unroll
+    for (int j = 0; j < kStreamChunk; ++j) {
+      const uint32_t l = l0 + j * kStep;
+      if (FULL || l < tsize) stream_store(reinterpret_cast<V *>(psi + (base | seg[l >> L] | (l & segmask))), v[j]);
+    }
   }
   __syncthreads();
 }
@@ -479,18 +625,95 @@ __device__ __forceinline__ void fill_segments(const TilePass &P, uint64_t *seg) 
   __syncthreads();
 }
 
-template <typename R, int RB>
+template <typename R, int RB, bool FULL>
 __global__ __launch_bounds__(kBlock) void k_tile_pass(amp_t<R> *__restrict__ psi, uint64_t n_tiles,
                                                       TilePass P, const TileStage *__restrict__ stages,
                                                       const RegOp *__restrict__ rops,
                                                       const TileOp *__restrict__ tops,
                                                       const amp_t<R> *__restrict__ mats) {
   extern __shared__ __align__(16) unsigned char smem[];
-  __shared__ uint64_t seg[1 << kTileMaxHigh];
+  // the tile starts at LDS address 0 (no static LDS), so tile offsets need no base add;
+  // the 2^h segment offsets follow the tile
   amp_t<R> *tile = reinterpret_cast<amp_t<R> *>(smem);
+  uint64_t *seg = reinterpret_cast<uint64_t *>(smem + (sizeof(amp_t<R>) << P.t));
   fill_segments(P, seg);
   for (uint64_t tix = blockIdx.x; tix < n_tiles; tix += gridDim.x)
-    tile_body<R, RB>(psi, tix, P, seg, tile, stages, rops, tops, mats);
+    tile_body<R, RB, FULL>(psi, tix, P, seg, tile, stages, rops, tops, mats);
+}
+
+// Software-pipelined tile pass: while the stages of tile i run out of LDS, the loads of the
+// workgroup's next tile are already in flight into registers, so HBM traffic overlaps the in-tile
+// arithmetic.  Used when a thread's share of a tile is exactly kStreamChunk 16-byte vectors
+// (2^t = kStreamChunk * kBlock * VN), so all streaming is unguarded straight-line code.
+template <typename R, int RB>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(sizeof(R) == 4 ? 4 : 1))) void k_tile_pass_pf(amp_t<R> *__restrict__ psi, uint64_t n_tiles,
+                                                         TilePass P, const TileStage *__restrict__ stages,
+                                                         const RegOp *__restrict__ rops,
+                                                         const TileOp *__restrict__ tops,
+                                                         const amp_t<R> *__restrict__ mats) {
+  using V = typename Vec16<R>::type;
+  constexpr int VN = Vec16<R>::n;
+  constexpr uint32_t kStep = kBlock * VN;
+  extern __shared__ __align__(16) unsigned char smem[];
+  // the tile starts at LDS address 0 (no static LDS), so tile offsets need no base add;
+  // the 2^h segment offsets follow the tile
+  amp_t<R> *tile = reinterpret_cast<amp_t<R> *>(smem);
+  uint64_t *seg = reinterpret_cast<uint64_t *>(smem + (sizeof(amp_t<R>) << P.t));
+  fill_segments(P, seg);
+  uint64_t tix = blockIdx.x;
+  if (tix >= n_tiles) return;  // uniform over the workgroup
+  const uint32_t t = P.t, L = P.L;
+  const uint32_t segmask = (1u << L) - 1;
+  const uint32_t l0 = threadIdx.x * VN;
+  const uint32_t s0 = lds_address(tile) + slot_bytes<R>(l0);  // l0 shares no bit with j * kStep
+  // Each turn of the loop runs the stages of the resident tile and stores it, then moves the
+  // registers holding the next tile (loaded during those stages) into LDS and starts loading the
+  // tile after.  At that wait only the older loads must be complete, not the stores just issued.
+  uint64_t base = tile_base(tix, P);
+  V v[kStreamChunk];
+#pragma unroll
+  for (int j = 0; j < kStreamChunk; ++j) {
+    const uint32_t l = l0 + j * kStep;
+    v[j] = stream_load(reinterpret_cast<const V *>(psi + (base | seg[l >> L] | (l & segmask))));
+  }
+#pragma unroll
+  for (int j = 0; j < kStreamChunk; ++j) tile_put16<R>(s0 ^ slot_bytes<R>(j * kStep), v[j]);
+  __syncthreads();
+  uint64_t next = tix + gridDim.x;
+  uint64_t nbase = tile_base(next < n_tiles ? next : tix, P);
+  auto prefetch = [&](uint64_t from) {
+    // the segment offsets are re-read from LDS for every tile rather than kept live across the
+    // stages (the empty asm keeps the compiler from hoisting them into registers)
+    uint32_t lq = l0;
+    asm volatile("" : "+v"(lq));
+#pragma unroll
+    for (int j = 0; j < kStreamChunk; ++j) {
+      const uint32_t l = lq + j * kStep;
+      v[j] = stream_load(reinterpret_cast<const V *>(psi + (from | seg[l >> L] | (l & segmask))));
+    }
+  };
+  if (next < n_tiles) prefetch(nbase);
+  while (true) {
+    run_stages<R, RB>(tile, t, base, P.n_stages, stages, rops, tops, mats);
+    uint32_t ls = l0;
+    asm volatile("" : "+v"(ls));
+#pragma unroll
+    for (int j = 0; j < kStreamChunk; ++j) {
+      const uint32_t l = ls + j * kStep;
+      stream_store(reinterpret_cast<V *>(psi + (base | seg[l >> L] | (l & segmask))),
+                   tile_get16<R>(s0 ^ slot_bytes<R>(j * kStep)));
+    }
+    __syncthreads();
+    if (next >= n_tiles) break;  // uniform over the workgroup
+#pragma unroll
+    for (int j = 0; j < kStreamChunk; ++j) tile_put16<R>(s0 ^ slot_bytes<R>(j * kStep), v[j]);
+    __syncthreads();
+    base = nbase;
+    const uint64_t cur = next;
+    next = cur + gridDim.x;
+    nbase = tile_base(next < n_tiles ? next : cur, P);
+    if (next < n_tiles) prefetch(nbase);
+  }
 }
 
 // Many circuits at once: workgroup b runs tile (b - job.first_block) of job block_job[b]; every
@@ -504,12 +727,12 @@ __global__ __launch_bounds__(kBlock) void k_batch_tile_pass(amp_t<R> *__restrict
                                                             const TileOp *__restrict__ tops,
                                                             const amp_t<R> *__restrict__ mats) {
   extern __shared__ __align__(16) unsigned char smem[];
-  __shared__ uint64_t seg[1 << kTileMaxHigh];
-  amp_t<R> *tile = reinterpret_cast<amp_t<R> *>(smem);
   const BatchJob *job = jobs + block_job[blockIdx.x];
-  const TilePass P = job->P;
+  const TilePass &P = job->P;
+  amp_t<R> *tile = reinterpret_cast<amp_t<R> *>(smem);
+  uint64_t *seg = reinterpret_cast<uint64_t *>(smem + (sizeof(amp_t<R>) << P.t));
   fill_segments(P, seg);
-  tile_body<R, RB>(psi + job->state_off, blockIdx.x - job->first_block, P, seg, tile, stages + job->stage_off,
+  tile_body<R, RB, false>(psi + job->state_off, blockIdx.x - job->first_block, P, seg, tile, stages + job->stage_off,
                    rops + job->rop_off, tops + job->top_off, mats + job->mat_off);
 }
 
@@ -1047,14 +1270,28 @@ template <typename R, int RB>
 void launch_tile_pass(amp_t<R> *psi, uint64_t n_tiles, const TilePass &P, const TileStage *d_stages,
                       const RegOp *d_rops, const TileOp *d_tops, const amp_t<R> *d_mats, hipStream_t s) {
   prepare_tile_kernels();
-  const size_t lds = sizeof(amp_t<R>) << P.t;
+  const size_t lds = (sizeof(amp_t<R>) << P.t) + (sizeof(uint64_t) << P.h);
   static const uint64_t cap = [] {
     const char *e = std::getenv("PROJECTQ_AMD_TILE_GRID");
     return e ? std::max<uint64_t>(1, std::strtoull(e, nullptr, 10)) : uint64_t(8192);
   }();
+  // the pipelined kernel is the default where it applies; PROJECTQ_AMD_TILE_PREFETCH=0 turns it off
+  static const bool prefetch = [] {
+    const char *e = std::getenv("PROJECTQ_AMD_TILE_PREFETCH");
+    return !e || std::atoi(e) != 0;
+  }();
   const uint64_t grid = std::min<uint64_t>(n_tiles, cap);
-  hipLaunchKernelGGL((k_tile_pass<R, RB>), dim3((unsigned)grid), dim3(kBlock), lds, s, psi, n_tiles, P,
-                     d_stages, d_rops, d_tops, d_mats);
+  const uint64_t chunk = uint64_t(kStreamChunk) * kBlock * Vec16<R>::n;  // amplitudes streamed per round
+  const uint64_t tsize = uint64_t(1) << P.t;
+  if (prefetch && tsize == chunk && RB == 4 && sizeof(R) == 4)
+    hipLaunchKernelGGL((k_tile_pass_pf<R, RB>), dim3((unsigned)grid), dim3(kBlock), lds, s, psi, n_tiles, P,
+                       d_stages, d_rops, d_tops, d_mats);
+  else if (tsize % chunk == 0)
+    hipLaunchKernelGGL((k_tile_pass<R, RB, true>), dim3((unsigned)grid), dim3(kBlock), lds, s, psi, n_tiles, P,
+                       d_stages, d_rops, d_tops, d_mats);
+  else
+    hipLaunchKernelGGL((k_tile_pass<R, RB, false>), dim3((unsigned)grid), dim3(kBlock), lds, s, psi, n_tiles, P,
+                       d_stages, d_rops, d_tops, d_mats);
   QS_HIP(hipGetLastError());
 }
 
@@ -1068,17 +1305,29 @@ void tile_pass(amp_t<R> *psi, uint64_t n_tiles, const TilePass &P, int rb, const
   else throw ValueError("tile pass: register bits must be 4 or 5");
 }
 
+// largest dynamic LDS of a tile kernel: the biggest tile plus the biggest segment table
+template <typename R>
+constexpr int kTileLdsMax = int(sizeof(amp_t<R>) << kTileMaxBits) + int(sizeof(uint64_t) << kTileMaxHigh);
+
 void prepare_tile_kernels() {
   static bool done = false;
   if (done) return;
-  QS_HIP(hipFuncSetAttribute((const void *)k_tile_pass<float, 4>, hipFuncAttributeMaxDynamicSharedMemorySize, 8 << kTileMaxBits));
-  QS_HIP(hipFuncSetAttribute((const void *)k_tile_pass<float, 5>, hipFuncAttributeMaxDynamicSharedMemorySize, 8 << kTileMaxBits));
-  QS_HIP(hipFuncSetAttribute((const void *)k_tile_pass<double, 4>, hipFuncAttributeMaxDynamicSharedMemorySize, 16 << kTileMaxBits));
-  QS_HIP(hipFuncSetAttribute((const void *)k_tile_pass<double, 5>, hipFuncAttributeMaxDynamicSharedMemorySize, 16 << kTileMaxBits));
-  QS_HIP(hipFuncSetAttribute((const void *)k_batch_tile_pass<float, 4>, hipFuncAttributeMaxDynamicSharedMemorySize, 8 << kTileMaxBits));
-  QS_HIP(hipFuncSetAttribute((const void *)k_batch_tile_pass<float, 5>, hipFuncAttributeMaxDynamicSharedMemorySize, 8 << kTileMaxBits));
-  QS_HIP(hipFuncSetAttribute((const void *)k_batch_tile_pass<double, 4>, hipFuncAttributeMaxDynamicSharedMemorySize, 16 << kTileMaxBits));
-  QS_HIP(hipFuncSetAttribute((const void *)k_batch_tile_pass<double, 5>, hipFuncAttributeMaxDynamicSharedMemorySize, 16 << kTileMaxBits));
+  QS_HIP(hipFuncSetAttribute((const void *)k_tile_pass<float, 4, true>, hipFuncAttributeMaxDynamicSharedMemorySize, kTileLdsMax<float>));
+  QS_HIP(hipFuncSetAttribute((const void *)k_tile_pass<float, 5, true>, hipFuncAttributeMaxDynamicSharedMemorySize, kTileLdsMax<float>));
+  QS_HIP(hipFuncSetAttribute((const void *)k_tile_pass<double, 4, true>, hipFuncAttributeMaxDynamicSharedMemorySize, kTileLdsMax<double>));
+  QS_HIP(hipFuncSetAttribute((const void *)k_tile_pass<double, 5, true>, hipFuncAttributeMaxDynamicSharedMemorySize, kTileLdsMax<double>));
+  QS_HIP(hipFuncSetAttribute((const void *)k_tile_pass<float, 4, false>, hipFuncAttributeMaxDynamicSharedMemorySize, kTileLdsMax<float>));
+  QS_HIP(hipFuncSetAttribute((const void *)k_tile_pass<float, 5, false>, hipFuncAttributeMaxDynamicSharedMemorySize, kTileLdsMax<float>));
+  QS_HIP(hipFuncSetAttribute((const void *)k_tile_pass<double, 4, false>, hipFuncAttributeMaxDynamicSharedMemorySize, kTileLdsMax<double>));
+  QS_HIP(hipFuncSetAttribute((const void *)k_tile_pass<double, 5, false>, hipFuncAttributeMaxDynamicSharedMemorySize, kTileLdsMax<double>));
+  QS_HIP(hipFuncSetAttribute((const void *)k_tile_pass_pf<float, 4>, hipFuncAttributeMaxDynamicSharedMemorySize, kTileLdsMax<float>));
+  QS_HIP(hipFuncSetAttribute((const void *)k_tile_pass_pf<float, 5>, hipFuncAttributeMaxDynamicSharedMemorySize, kTileLdsMax<float>));
+  QS_HIP(hipFuncSetAttribute((const void *)k_tile_pass_pf<double, 4>, hipFuncAttributeMaxDynamicSharedMemorySize, kTileLdsMax<double>));
+  QS_HIP(hipFuncSetAttribute((const void *)k_tile_pass_pf<double, 5>, hipFuncAttributeMaxDynamicSharedMemorySize, kTileLdsMax<double>));
+  QS_HIP(hipFuncSetAttribute((const void *)k_batch_tile_pass<float, 4>, hipFuncAttributeMaxDynamicSharedMemorySize, kTileLdsMax<float>));
+  QS_HIP(hipFuncSetAttribute((const void *)k_batch_tile_pass<float, 5>, hipFuncAttributeMaxDynamicSharedMemorySize, kTileLdsMax<float>));
+  QS_HIP(hipFuncSetAttribute((const void *)k_batch_tile_pass<double, 4>, hipFuncAttributeMaxDynamicSharedMemorySize, kTileLdsMax<double>));
+  QS_HIP(hipFuncSetAttribute((const void *)k_batch_tile_pass<double, 5>, hipFuncAttributeMaxDynamicSharedMemorySize, kTileLdsMax<double>));
   done = true;
 }
 
@@ -1095,7 +1344,7 @@ void launch_batch_tile_pass(amp_t<R> *psi, const BatchJob *jobs, const uint32_t 
                             uint32_t max_t, const TileStage *stages, const RegOp *rops, const TileOp *tops,
                             const amp_t<R> *mats, hipStream_t s) {
   prepare_tile_kernels();
-  hipLaunchKernelGGL((k_batch_tile_pass<R, RB>), dim3(n_blocks), dim3(kBlock), sizeof(amp_t<R>) << max_t, s, psi,
+  hipLaunchKernelGGL((k_batch_tile_pass<R, RB>), dim3(n_blocks), dim3(kBlock), (sizeof(amp_t<R>) << max_t) + (sizeof(uint64_t) << kTileMaxHigh), s, psi,
                      jobs, block_job, stages, rops, tops, mats);
   QS_HIP(hipGetLastError());
 }

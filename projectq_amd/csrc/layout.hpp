@@ -42,9 +42,12 @@ struct TileStage {
   uint32_t kind;        // kStageRegisters or kStageLds
   uint32_t begin, end;  // RegOp index range (register stage) or the TileOp index (LDS stage)
   uint32_t fixed[5];    // register stage: tile-local position of register bit i, ascending
+  uint32_t soff[32];    // register stage: LDS byte offset slot(index of register i), i < 2^RB
 };
 
-// One op of a register stage.  code = kind << 8 | b1 << 4 | b0 with b0 (< b1) register bits:
+// One op of a register stage.  code = cb << 12 | kind << 8 | b1 << 4 | b0 with b0 (< b1)
+// register bits and cb = 1 + the op's one control among the register bits (0: none; an op with
+// two or more register-bit controls runs as an LDS op instead):
 //   kind 0: 1-qubit general 2x2 on b0      kind 3: 1-qubit X (swap) on b0
 //   kind 1: 1-qubit diagonal on b0         kind 4: 2-qubit general 4x4 on (b0, b1)
 //   kind 2: 1-qubit anti-diagonal on b0    kind 5: 2-qubit diagonal on (b0, b1)
@@ -61,7 +64,7 @@ enum : uint32_t {
 };
 struct RegOp {
   uint32_t code;
-  uint32_t rcmask;  // controls among the stage's register bits (register-index mask)
+  uint32_t rcmask;  // controls among the stage's register bits (register-index mask, <= 1 bit)
   uint32_t lcmask;  // tile-local controls outside the stage's register bits
   uint32_t mat;     // first element of the op's matrix in the pass's matrix array
   uint64_t gcmask;  // controls outside the tile
@@ -94,11 +97,67 @@ struct EnergyJob {
   uint32_t term_off, n_terms;
 };
 
-// LDS address of tile-local index x: an XOR swizzle of the low bits by the next bits, so that
-// threads reading with power-of-two strides spread over the banks.  Pairs (2j, 2j+1) stay
-// adjacent for complex64 so 16-byte vector accesses still work.  A bijection on [0, 2^t).
-inline constexpr uint32_t lds_slot_c64(uint32_t x) { return x ^ (((x >> 5) & 15u) << 1); }
-inline constexpr uint32_t lds_slot_c128(uint32_t x) { return x ^ ((x >> 4) & 15u); }
+// LDS slot of tile-local index x: an XOR swizzle that spreads the accesses of a register stage
+// over the LDS banks.  In a register stage the lanes of a wave vary over the lowest tile bits
+// that are not register bits, so the bank of a lane is a GF(2)-linear function of those bits;
+// the swizzle adds to the low (bank) bits a mask for every set bit b >= 4 (c64) / >= 3 (c128).
+// gfx950 LDS has 64 banks of 4 bytes; it serves 32 lanes of 8-byte accesses (bank pair = 8-byte
+// unit mod 32) or 16 lanes of 16-byte accesses (bank quad = 16-byte unit mod 16) per cycle.
+// The masks were picked by a search over all 495 four-bit register sets of a 12-bit tile,
+// minimising the mean conflict degree of those accesses (c64: 8-byte amplitudes, c128: 16-byte).  Bit b only flips bits below min(b, 5), so
+// the map is unitriangular: linear, a bijection on [0, 2^t), and slot(a | b) = slot(a) ^ slot(b)
+// for indices without common bits.  complex64 pairs (2j, 2j+1) stay adjacent but may swap.
+inline constexpr uint32_t lds_slot_c64(uint32_t x) {
+  return x ^ (((x >> 4) & 1u) * 11u) ^ (((x >> 5) & 1u) * 30u) ^ (((x >> 6) & 1u) * 15u) ^
+         (((x >> 7) & 1u) * 23u) ^ (((x >> 8) & 1u) * 16u) ^ (((x >> 9) & 1u) * 3u) ^
+         (((x >> 10) & 1u) * 31u) ^ (((x >> 11) & 1u) * 23u);
+}
+inline constexpr uint32_t lds_slot_c128(uint32_t x) {
+  return x ^ (((x >> 3) & 1u) * 6u) ^ (((x >> 4) & 1u) * 13u) ^ (((x >> 5) & 1u) * 7u) ^
+         (((x >> 6) & 1u) * 11u) ^ (((x >> 7) & 1u) * 8u) ^ (((x >> 8) & 1u) * 12u) ^
+         (((x >> 10) & 1u) * 11u) ^ (((x >> 11) & 1u) * 4u);
+}
+
+// Bank-conflict degree of a register stage under that model: the lowest `lanes_bits`
+// non-register tile bits vary across the lanes served together (5 for complex64, 4 for
+// complex128), and the bank of a lane is the low `lanes_bits` bits of its slot.
+// Returns 2^(lanes_bits - rank).
+inline uint32_t stage_conflict_degree(uint32_t reg_bits_mask, uint32_t t, bool c64, uint32_t lanes_bits) {
+  uint32_t rows[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  uint32_t n = 0;
+  const uint32_t bank_mask = (1u << lanes_bits) - 1;
+  for (uint32_t b = 0; b < t && n < lanes_bits; ++b)
+    if (!((reg_bits_mask >> b) & 1)) rows[n++] = (c64 ? lds_slot_c64(1u << b) : lds_slot_c128(1u << b)) & bank_mask;
+  uint32_t rank = 0;
+  for (uint32_t bit = 0; bit < lanes_bits; ++bit) {
+    uint32_t piv = n;
+    for (uint32_t i = rank; i < n; ++i)
+      if ((rows[i] >> bit) & 1) {
+        piv = i;
+        break;
+      }
+    if (piv == n) continue;
+    const uint32_t tmp = rows[rank];
+    rows[rank] = rows[piv];
+    rows[piv] = tmp;
+    for (uint32_t i = 0; i < n; ++i)
+      if (i != rank && ((rows[i] >> bit) & 1)) rows[i] ^= rows[rank];
+    ++rank;
+  }
+  return 1u << (lanes_bits - rank);
+}
+
+// Fill the register offsets of a register stage whose `rb` register bits are in st.fixed: the
+// LDS byte offset of the tile index made of register bits i (i < 2^rb), so that the kernel
+// addresses register i of a thread as slot_bytes(lb) ^ soff[i].
+inline void fill_stage_offsets(TileStage &st, uint32_t rb, bool c64) {
+  for (uint32_t i = 0; i < 32; ++i) {
+    uint32_t o = 0;
+    for (uint32_t b = 0; b < rb; ++b)
+      if ((i >> b) & 1) o |= 1u << st.fixed[b];
+    st.soff[i] = i < (1u << rb) ? (c64 ? lds_slot_c64(o) * 8u : lds_slot_c128(o) * 16u) : 0u;
+  }
+}
 
 }  // namespace dev
 }  // namespace qs

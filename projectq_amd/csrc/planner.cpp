@@ -200,9 +200,46 @@ void emit_lds_op(const LocalGate &g, unsigned t, TileProgram &prog) {
   prog.stages.push_back(st);
 }
 
-void emit_reg_stage(const std::vector<LocalGate> &run, uint32_t bits, unsigned t, unsigned rb, TileProgram &prog) {
-  // pad the stage's register bits up to rb with the lowest other tile bits
-  for (uint32_t p = 0; p < t && popcount32(bits) < rb; ++p) bits |= 1u << p;
+// Register bits of a stage: the op targets `need` plus padding up to rb bits.  The padding is the
+// choice with the fewest LDS bank conflicts (stage_conflict_degree over the lanes one LDS cycle
+// serves: 32 lanes of 8-byte accesses for complex64, 16 lanes of 16-byte accesses for
+// complex128), ties going to the lowest bits.
+uint32_t pad_stage_bits(uint32_t need, unsigned t, unsigned rb, bool c64) {
+  const uint32_t have = (uint32_t)popcount32(need);
+  if (have >= rb || t <= rb) {
+    uint32_t bits = need;
+    for (uint32_t p = 0; p < t && (uint32_t)popcount32(bits) < rb; ++p) bits |= 1u << p;
+    return bits;
+  }
+  std::vector<uint32_t> free_bits;
+  for (uint32_t p = 0; p < t; ++p)
+    if (!((need >> p) & 1)) free_bits.push_back(p);
+  const uint32_t k = rb - have;
+  const uint32_t lane_bits = c64 ? 5u : 4u;
+  uint32_t best = 0, best_cost = ~0u;
+  // enumerate k-subsets of free_bits in lexicographic order (lowest bits first)
+  std::vector<uint32_t> idx(k);
+  for (uint32_t i = 0; i < k; ++i) idx[i] = i;
+  while (true) {
+    uint32_t bits = need;
+    for (uint32_t i : idx) bits |= 1u << free_bits[i];
+    const uint32_t cost = dev::stage_conflict_degree(bits, t, c64, lane_bits);
+    if (cost < best_cost) {
+      best_cost = cost;
+      best = bits;
+    }
+    int i = (int)k - 1;
+    while (i >= 0 && idx[i] == free_bits.size() - k + (uint32_t)i) --i;
+    if (i < 0) break;
+    ++idx[i];
+    for (uint32_t j = (uint32_t)i + 1; j < k; ++j) idx[j] = idx[j - 1] + 1;
+  }
+  return best;
+}
+
+void emit_reg_stage(const std::vector<LocalGate> &run, uint32_t bits, unsigned t, unsigned rb, bool c64,
+                    TileProgram &prog) {
+  bits = pad_stage_bits(bits, t, rb, c64);
   dev::TileStage st{};
   st.kind = dev::kStageRegisters;
   int regbit[32];
@@ -214,7 +251,13 @@ void emit_reg_stage(const std::vector<LocalGate> &run, uint32_t bits, unsigned t
       st.fixed[nb++] = p;
     }
   if (nb != rb) throw std::runtime_error("tile pass: register stage width");
+  dev::fill_stage_offsets(st, rb, c64);
   st.begin = (uint32_t)prog.rops.size();
+  auto close_stage = [&]() {
+    st.end = (uint32_t)prog.rops.size();
+    if (st.end > st.begin) prog.stages.push_back(st);
+    st.begin = st.end;
+  };
   for (const LocalGate &g : run) {
     dev::RegOp op{};
     op.gcmask = g.gc;
@@ -223,10 +266,18 @@ void emit_reg_stage(const std::vector<LocalGate> &run, uint32_t bits, unsigned t
       if (regbit[p] >= 0) op.rcmask |= 1u << regbit[p];
       else op.lcmask |= 1u << p;
     }
+    if (popcount32(op.rcmask) > 1) {
+      // the register kernels take at most one register-bit control: run this gate on the LDS
+      // tile between the ops before it and the ops after it
+      close_stage();
+      emit_lds_op(g, t, prog);
+      continue;
+    }
+    const uint32_t cb = op.rcmask ? 1u + (uint32_t)__builtin_ctz(op.rcmask) : 0u;
     op.mat = (uint32_t)prog.mats.size();
     if (g.k == 1) {
       const uint32_t b0 = (uint32_t)regbit[g.tb[0]];
-      op.code = classify1(*g.m) << 8 | b0;
+      op.code = cb << 12 | classify1(*g.m) << 8 | b0;
       prog.mats.insert(prog.mats.end(), g.m->begin(), g.m->end());
     } else {
       uint32_t b0 = (uint32_t)regbit[g.tb[0]], b1 = (uint32_t)regbit[g.tb[1]];
@@ -239,18 +290,18 @@ void emit_reg_stage(const std::vector<LocalGate> &run, uint32_t bits, unsigned t
         m.swap(w);
         std::swap(b0, b1);
       }
-      op.code = (is_diag4(m) ? dev::kOp2Diag : dev::kOp2General) << 8 | b1 << 4 | b0;
+      op.code = cb << 12 | (is_diag4(m) ? dev::kOp2Diag : dev::kOp2General) << 8 | b1 << 4 | b0;
       prog.mats.insert(prog.mats.end(), m.begin(), m.end());
     }
     prog.rops.push_back(op);
   }
-  st.end = (uint32_t)prog.rops.size();
-  prog.stages.push_back(st);
+  close_stage();
 }
 
 }  // namespace
 
-void build_tile_pass(const std::vector<PGate> &gates, const PPass &ps, unsigned t, unsigned rb, TileProgram &prog) {
+void build_tile_pass(const std::vector<PGate> &gates, const PPass &ps, unsigned t, unsigned rb, bool c64,
+                     TileProgram &prog) {
   prog = TileProgram{};
   dev::TilePass &P = prog.P;
   P.t = t;
@@ -266,6 +317,8 @@ void build_tile_pass(const std::vector<PGate> &gates, const PPass &ps, unsigned 
     tilepos.push_back(p);
   }
   for (uint32_t j = 0; j < P.h; ++j) {
+    if (ps.high[j] < P.L || (j > 0 && ps.high[j] <= ps.high[j - 1]))
+      throw std::runtime_error("tile planner: high tile positions must ascend above the low bits");
     P.hpos[j] = ps.high[j];
     loc[ps.high[j]] = (int)(P.L + j);
     tilepos.push_back(ps.high[j]);
@@ -277,7 +330,7 @@ void build_tile_pass(const std::vector<PGate> &gates, const PPass &ps, unsigned 
   std::vector<LocalGate> run;
   uint32_t run_bits = 0;
   auto flush = [&]() {
-    if (!run.empty()) emit_reg_stage(run, run_bits, t, rb, prog);
+    if (!run.empty()) emit_reg_stage(run, run_bits, t, rb, c64, prog);
     run.clear();
     run_bits = 0;
   };
@@ -328,7 +381,7 @@ PassFeatures tile_pass_features(const TileProgram &prog, unsigned n, bool c64, u
   float reg = 0, lds = 0, kinds[7] = {0, 0, 0, 0, 0, 0, 0}, rctl = 0, lctl = 0, gctl = 0, ldsflops = 0;
   for (const auto &st : prog.stages) (st.kind == dev::kStageRegisters ? reg : lds) += 1;
   for (const auto &op : prog.rops) {
-    kinds[std::min<uint32_t>(op.code >> 8, 6)] += 1;
+    kinds[std::min<uint32_t>((op.code >> 8) & 15u, 6)] += 1;
     rctl += op.rcmask != 0;
     lctl += op.lcmask != 0;
     gctl += op.gcmask != 0;

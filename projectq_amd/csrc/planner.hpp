@@ -76,7 +76,7 @@ struct TileProgram {
   std::vector<dev::TileOp> tops;
   std::vector<std::complex<double>> mats;
 };
-void build_tile_pass(const std::vector<PGate> &gates, const PPass &pass, unsigned t, unsigned rb,
+void build_tile_pass(const std::vector<PGate> &gates, const PPass &pass, unsigned t, unsigned rb, bool c64,
                      TileProgram &prog);
 
 // A Pauli string applied left to right, ops given as (bit position, 'X'|'Y'|'Z'), equals

@@ -34,8 +34,12 @@ void check(bool ok, const char *what) {
 
 uint32_t slot(uint32_t x, bool c64) { return c64 ? qs::dev::lds_slot_c64(x) : qs::dev::lds_slot_c128(x); }
 
-void apply_small(std::vector<cplx> &v, uint32_t code, uint32_t rc, const cplx *m, unsigned rb) {
-  const uint32_t kind = code >> 8, b0 = code & 15u, b1 = (code >> 4) & 15u;
+void apply_small(std::vector<cplx> &v, uint32_t code, uint32_t rcmask, const cplx *m, unsigned rb) {
+  const uint32_t kind = (code >> 8) & 15u, b0 = code & 15u, b1 = (code >> 4) & 15u, cb = code >> 12;
+  // the kernels read the register control from the code; rcmask must agree with it
+  check(cb <= rb && (cb == 0 ? rcmask == 0 : rcmask == (1u << (cb - 1))), "register control encoding");
+  check(cb == 0 || (cb - 1 != b0 && (kind < 4 || kind == qs::dev::kOp1Real || cb - 1 != b1)), "control on a target");
+  const uint32_t rc = cb ? 1u << (cb - 1) : 0u;
   const uint32_t NR = 1u << rb;
   check(b0 < rb && (kind < 4 || kind == qs::dev::kOp1Real || (b1 < rb && b0 < b1)), "register op bits");
   if (kind < 4 || kind == qs::dev::kOp1Real) {
@@ -103,7 +107,14 @@ void replay(cplx *psi, unsigned n, const qs::TileProgram &prog, unsigned rb, boo
     return tile[y];
   };
   for (uint64_t tix = 0; tix < n_tiles; ++tix) {
-    const uint64_t base = insert_zeros(tix, P.tiles);
+    // the kernel's tile_base: tix above the low L bits, zeros inserted at the ascending high bits
+    uint64_t base = tix << L;
+    for (uint32_t b = 0; b < h; ++b) {
+      check(P.hpos[b] >= L && (b == 0 || P.hpos[b] > P.hpos[b - 1]), "high bits ascend above L");
+      const uint32_t p = P.hpos[b];
+      base = ((base >> p) << (p + 1)) | (base & ((uint64_t(1) << p) - 1));
+    }
+    check(base == insert_zeros(tix, P.tiles), "tile base agrees with the tile bit list");
     check(base < N, "tile base");
     for (uint32_t l = 0; l < tsize; ++l) {
       check((l >> L) < seg.size(), "segment index");
@@ -120,6 +131,15 @@ void replay(cplx *psi, unsigned n, const qs::TileProgram &prog, unsigned rb, boo
           for (uint32_t b = 0; b < rb; ++b)
             if ((i >> b) & 1) roff[i] |= 1u << st.fixed[b];
         for (uint32_t b = 1; b < rb; ++b) check(st.fixed[b - 1] < st.fixed[b] && st.fixed[b] < t, "stage bits");
+        // the kernel addresses register i as slot(lb) ^ soff[i] (bytes): the planner's offsets
+        // must be the swizzled register indices
+        const uint32_t amp_bytes = c64 ? 8u : 16u;
+        for (uint32_t i = 0; i < NR; ++i) check(st.soff[i] == slot(roff[i], c64) * amp_bytes, "stage offsets");
+        auto reg_slot = [&](uint32_t lb, uint32_t i) -> cplx & {
+          const uint32_t y = slot(lb, c64) ^ (st.soff[i] / amp_bytes);
+          check(y < tsize && y == slot(lb | roff[i], c64), "register slot");
+          return tile[y];
+        };
         std::vector<cplx> v(NR);
         for (uint32_t g = 0; g < (1u << (t - rb)); ++g) {
           uint32_t lb = g;
@@ -127,17 +147,17 @@ void replay(cplx *psi, unsigned n, const qs::TileProgram &prog, unsigned rb, boo
             uint32_t p = st.fixed[b];
             lb = ((lb >> p) << (p + 1)) | (lb & ((1u << p) - 1));
           }
-          for (uint32_t i = 0; i < NR; ++i) v[i] = lds(lb | roff[i]);
+          for (uint32_t i = 0; i < NR; ++i) v[i] = reg_slot(lb, i);
           for (uint32_t o = st.begin; o < st.end; ++o) {
             const auto &op = prog.rops[o];
             if ((base & op.gcmask) != op.gcmask) continue;
             if ((lb & op.lcmask) != op.lcmask) continue;
-            const uint32_t kd = op.code >> 8;
+            const uint32_t kd = (op.code >> 8) & 15u;
             const uint32_t width = (kd < 4 || kd == qs::dev::kOp1Real) ? 4 : 16;
             check(op.mat + width <= prog.mats.size(), "matrix offset");
             apply_small(v, op.code, op.rcmask, prog.mats.data() + op.mat, rb);
           }
-          for (uint32_t i = 0; i < NR; ++i) lds(lb | roff[i]) = v[i];
+          for (uint32_t i = 0; i < NR; ++i) reg_slot(lb, i) = v[i];
         }
       } else {
         check(st.begin < prog.tops.size(), "LDS op index");
@@ -168,27 +188,72 @@ void replay(cplx *psi, unsigned n, const qs::TileProgram &prog, unsigned rb, boo
   }
 }
 
+std::vector<qs::PGate> read_gates(int n_gates, const uint32_t *k, const uint32_t *tpos, const uint64_t *cmask,
+                                   const double *mats) {
+  std::vector<qs::PGate> gates;
+  size_t to = 0, mo = 0;
+  for (int i = 0; i < n_gates; ++i) {
+    qs::PGate g;
+    for (uint32_t l = 0; l < k[i]; ++l) g.tpos.push_back(tpos[to++]);
+    size_t d = size_t(1) << k[i];
+    for (size_t e = 0; e < d * d; ++e, ++mo) g.m.push_back(cplx(mats[2 * mo], mats[2 * mo + 1]));
+    g.cmask = cmask[i];
+    if (g.tmask() & g.cmask) continue;
+    gates.push_back(g);
+  }
+  return gates;
+}
+
 }  // namespace
 
 extern "C" {
 
+// The LDS swizzle the kernels use (layout.hpp), for the CPU tests.
+uint32_t emu_slot(uint32_t x, int c64) { return slot(x, c64 != 0); }
+
+// Planning only (no state): passes, register stages, LDS stages and register ops of the tile
+// plan of a gate list, for checks on how compact the planner's passes are.
+int emu_plan_stats(unsigned n, unsigned t, unsigned lmin, unsigned rb, int c64, int n_gates, const uint32_t *k,
+                   const uint32_t *tpos, const uint64_t *cmask, const double *mats, int *stats,
+                   uint32_t *stage_bits, int max_stage_bits, char *err, int err_len) {
+  try {
+    auto gates = qs::merge_single_qubit_runs(read_gates(n_gates, k, tpos, cmask, mats));
+    auto passes = qs::schedule_tiles(gates, qs::TileShape{n, t, lmin});
+    int reg = 0, lds = 0, rops = 0;
+    for (auto &ps : passes) {
+      qs::TileProgram prog;
+      qs::build_tile_pass(gates, ps, std::min(t, n), rb, c64 != 0, prog);
+      for (const auto &st : prog.stages) (st.kind == qs::dev::kStageRegisters ? reg : lds) += 1;
+      rops += (int)prog.rops.size();
+    }
+    if (stage_bits) {  // optional: the register bits of every register stage, rb per stage
+      int w = 0;
+      for (auto &ps : passes) {
+        qs::TileProgram prog;
+        qs::build_tile_pass(gates, ps, std::min(t, n), rb, c64 != 0, prog);
+        for (const auto &st : prog.stages)
+          if (st.kind == qs::dev::kStageRegisters && w + (int)rb <= max_stage_bits)
+            for (unsigned b = 0; b < rb; ++b) stage_bits[w++] = st.fixed[b];
+      }
+    }
+    stats[0] = (int)passes.size();
+    stats[1] = reg;
+    stats[2] = lds;
+    stats[3] = rops;
+    stats[4] = (int)gates.size();
+    return 0;
+  } catch (const std::exception &e) {
+    std::strncpy(err, e.what(), err_len - 1);
+    return 2;
+  }
+}
+
 // gates: k[i] targets at tpos[off..off+k), controls cmask[i], matrix at mats[moff..] (complex
 // interleaved, 4^k entries).  state: 2^n complex interleaved, updated in place.
-int emu_run(unsigned n, unsigned t, unsigned lmin, unsigned rb, int n_gates, const uint32_t *k, const uint32_t *tpos,
+int emu_run(unsigned n, unsigned t, unsigned lmin, unsigned rb, int c64, int n_gates, const uint32_t *k, const uint32_t *tpos,
             const uint64_t *cmask, const double *mats, double *state, int *n_passes, char *err, int err_len) {
   try {
-    std::vector<qs::PGate> gates;
-    size_t to = 0, mo = 0;
-    for (int i = 0; i < n_gates; ++i) {
-      qs::PGate g;
-      for (uint32_t l = 0; l < k[i]; ++l) g.tpos.push_back(tpos[to++]);
-      size_t d = size_t(1) << k[i];
-      for (size_t e = 0; e < d * d; ++e, ++mo) g.m.push_back(cplx(mats[2 * mo], mats[2 * mo + 1]));
-      g.cmask = cmask[i];
-      if (g.tmask() & g.cmask) continue;
-      gates.push_back(g);
-    }
-    gates = qs::merge_single_qubit_runs(gates);
+    auto gates = qs::merge_single_qubit_runs(read_gates(n_gates, k, tpos, cmask, mats));
     auto passes = qs::schedule_tiles(gates, qs::TileShape{n, t, lmin});
     size_t covered = 0;
     if (!state) {  // planning only: count passes
@@ -197,8 +262,8 @@ int emu_run(unsigned n, unsigned t, unsigned lmin, unsigned rb, int n_gates, con
     }
     for (auto &ps : passes) {
       qs::TileProgram prog;
-      qs::build_tile_pass(gates, ps, std::min(t, n), rb, prog);
-      replay(reinterpret_cast<cplx *>(state), n, prog, rb, rb == 5);
+      qs::build_tile_pass(gates, ps, std::min(t, n), rb, c64 != 0, prog);
+      replay(reinterpret_cast<cplx *>(state), n, prog, rb, c64 != 0);
       covered += ps.gates.size();
     }
     check(covered == gates.size(), "every gate scheduled exactly once");

@@ -34,7 +34,7 @@ def emu():
     return lib
 
 
-def run_emu(lib, n, t, lmin, cmds, state, rb=5):
+def run_emu(lib, n, t, lmin, cmds, state, rb=5, c64=True):
     ks, tpos, cm, mats = [], [], [], []
     for c in cmds:
         if c[0] != "gate":
@@ -59,6 +59,7 @@ def run_emu(lib, n, t, lmin, cmds, state, rb=5):
         ctypes.c_uint(t),
         ctypes.c_uint(lmin),
         ctypes.c_uint(rb),
+        ctypes.c_int(int(c64)),
         ctypes.c_int(len(ks)),
         k.ctypes.data_as(ctypes.c_void_p),
         tp.ctypes.data_as(ctypes.c_void_p),
@@ -93,11 +94,15 @@ CASES = [
 
 
 @pytest.mark.parametrize("name,make", CASES, ids=[c[0] for c in CASES])
-@pytest.mark.parametrize("n,t,lmin,rb", [(8, 8, 5, 5), (12, 10, 4, 4), (14, 10, 5, 5), (15, 12, 4, 4), (16, 13, 5, 5), (4, 4, 4, 5)])
-def test_tile_plan_matches_unfused(emu, name, make, n, t, lmin, rb):
+@pytest.mark.parametrize(
+    "n,t,lmin,rb,c64",
+    [(8, 8, 5, 5, True), (12, 10, 4, 4, True), (14, 10, 5, 5, False), (15, 12, 4, 4, True), (15, 12, 4, 4, False),
+     (16, 13, 5, 5, True), (4, 4, 4, 5, False)],
+)
+def test_tile_plan_matches_unfused(emu, name, make, n, t, lmin, rb, c64):
     cmds = make(n)
     want = oracle_state(cmds, n)
-    got, npass = run_emu(emu, n, t, lmin, cmds, start_state(n), rb)
+    got, npass = run_emu(emu, n, t, lmin, cmds, start_state(n), rb, c64)
     assert np.max(np.abs(got - want)) <= 1e-12
     n_gates = commands.count_gates(cmds)
     assert 1 <= npass <= n_gates
@@ -115,8 +120,9 @@ def test_random_controlled_blocks(emu):
         cmds.append(commands.gate(u, [int(q) for q in picks[:k]], [int(q) for q in picks[k:]]))
     want = oracle_state(cmds, n)
     for rb in (4, 5):
-        got, _ = run_emu(emu, n, 10, 4, cmds, start_state(n), rb)
-        assert np.max(np.abs(got - want)) <= 1e-12
+        for c64 in (True, False):
+            got, _ = run_emu(emu, n, 10, 4, cmds, start_state(n), rb, c64)
+            assert np.max(np.abs(got - want)) <= 1e-12
 
 
 def test_pass_count_random32_layer(emu):
@@ -128,3 +134,82 @@ def test_pass_count_random32_layer(emu):
     want = oracle_state(cmds, n)
     assert np.max(np.abs(got - want)) <= 1e-12
     assert npass <= 4 * 3
+
+
+def plan_stage_bits(lib, n, t, lmin, rb, cmds, c64):
+    ks, tpos, cm, mats = [], [], [], []
+    for c in cmds:
+        if c[0] != "gate":
+            continue
+        _, m, targets, controls = c
+        ks.append(len(targets))
+        tpos.extend(targets)
+        mask = 0
+        for q in controls:
+            mask |= 1 << q
+        cm.append(mask)
+        mats.append(np.asarray(m, dtype=np.complex128).reshape(-1))
+    k = np.asarray(ks, dtype=np.uint32)
+    tp = np.asarray(tpos, dtype=np.uint32)
+    cmask = np.asarray(cm, dtype=np.uint64)
+    mflat = np.ascontiguousarray(np.concatenate(mats))
+    stats = np.zeros(5, dtype=np.int32)
+    bits = np.zeros(200000, dtype=np.uint32)
+    err = ctypes.create_string_buffer(256)
+    rc = lib.emu_plan_stats(
+        ctypes.c_uint(n), ctypes.c_uint(t), ctypes.c_uint(lmin), ctypes.c_uint(rb), ctypes.c_int(int(c64)),
+        ctypes.c_int(len(ks)), k.ctypes.data_as(ctypes.c_void_p), tp.ctypes.data_as(ctypes.c_void_p),
+        cmask.ctypes.data_as(ctypes.c_void_p), mflat.ctypes.data_as(ctypes.c_void_p),
+        stats.ctypes.data_as(ctypes.c_void_p), bits.ctypes.data_as(ctypes.c_void_p), ctypes.c_int(len(bits)), err,
+        ctypes.c_int(256),
+    )
+    assert rc == 0, err.value.decode()
+    return stats, [tuple(int(b) for b in bits[i * rb:(i + 1) * rb]) for i in range(stats[1])]
+
+
+@pytest.mark.parametrize("c64", [True, False])
+def test_lds_swizzle_is_a_linear_bijection(emu, c64):
+    emu.emu_slot.restype = ctypes.c_uint32
+    slot = lambda x: emu.emu_slot(ctypes.c_uint32(x), ctypes.c_int(int(c64)))  # noqa: E731
+    image = [slot(x) for x in range(1 << 13)]
+    assert sorted(image) == list(range(1 << 13))
+    rng = np.random.default_rng(7)
+    for _ in range(300):
+        a = int(rng.integers(1 << 13))
+        b = int(rng.integers(1 << 13)) & ~a
+        assert slot(a | b) == slot(a) ^ slot(b)
+    if c64:  # pairs (2j, 2j + 1) stay adjacent: tiles stream 16-byte pairs as two 8-byte halves
+        assert all(slot(2 * j) ^ slot(2 * j + 1) == 1 for j in range(1 << 12))
+
+
+def bank_conflict_degree(slot, stage, t, lanes_per_cycle, n_banks):
+    """Brute force: serve the 256 threads of a register stage lanes_per_cycle at a time and count
+    the worst number of lanes that hit one bank, averaged over accesses."""
+    nonstage = [b for b in range(t) if b not in stage]
+    total = count = 0
+    for i in range(1 << len(stage)):
+        roff = sum(1 << stage[b] for b in range(len(stage)) if (i >> b) & 1)
+        for g0 in range(0, 256, lanes_per_cycle):
+            hits = {}
+            for g in range(g0, g0 + lanes_per_cycle):
+                lb = sum(1 << nonstage[k] for k in range(len(nonstage)) if (g >> k) & 1)
+                bank = slot(lb | roff) % n_banks
+                hits[bank] = hits.get(bank, 0) + 1
+            total += max(hits.values())
+            count += 1
+    return total / count
+
+
+def test_register_stages_spread_over_lds_banks(emu):
+    """The planner's register stages for 20 layers of the 32-qubit random circuit (t=12, RB=4,
+    complex64) read and write LDS with few bank conflicts.  gfx950 LDS serves 32 lanes of 8-byte
+    accesses per cycle over 32 bank pairs (64 banks of 4 bytes); brute force over every access
+    gives the mean worst-case lanes per bank pair.  The swizzle this one replaced: 1.76."""
+    emu.emu_slot.restype = ctypes.c_uint32
+    slot = lambda x: emu.emu_slot(ctypes.c_uint32(x), ctypes.c_int(1))  # noqa: E731
+    cmds = circuits.random_layers(32, 20, 16, 2024, measure=False)
+    stats, stages = plan_stage_bits(emu, 32, 12, 4, 4, cmds, True)
+    assert stats[1] == len(stages) > 0
+    degrees = {s: bank_conflict_degree(slot, s, 12, 32, 32) for s in set(stages)}
+    mean = sum(degrees[s] for s in stages) / len(stages)
+    assert mean <= 1.15, mean
