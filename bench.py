@@ -11,7 +11,10 @@ Also reported:
   timed on a few layers of the same circuit, and the speed-up of the headline over it;
 * roofline of the dominant kernel (the gate pass): algorithmic bytes / summed HIP-event time;
 * a CPU baseline: ProjectQ's own C++ simulator built from its sources (oracle/_ref), or the
-  in-repo oracle where that build is absent, on a bounded sample of the same circuit.
+  in-repo oracle where that build is absent, on a bounded sample of the same circuit;
+* batched_variational: 512 variational ansatze with 100-term Pauli energies in one hipGraph launch;
+* qft_rotations_adder_30q: the 30-qubit QFT + rotations + AddQuantum + probabilities + measurement
+  workload in complex128, with bit-exact and normalisation checks.
 """
 
 import argparse
@@ -187,6 +190,94 @@ def batched_variational(args):
     }
 
 
+def qft_rotations_adder(args, n=30):
+    """The 30-qubit QFT + rotations + AddQuantum workload, complex128 (the reference's precision):
+    QFT (30 H + 435 controlled phases), 50 layers of a random Rx/Ry/Rz per qubit plus 15
+    controlled-Rz with 1-3 random controls, b <- (a + b) mod 2^15 on the two 15-qubit halves through
+    emulate_math, 8 probabilities of 8-qubit patterns, seeded measurement of all 30 qubits.
+    Window fusion with the learned planner.  Checks that need no reference at this size: the
+    adder moves 64 sampled amplitudes bit-exactly to (a, (a + b) mod 2^15), and P(q=0) + P(q=1) = 1.
+    The plain path (run() after every gate) is timed on the first 60 gates and scaled."""
+    import numpy as np
+
+    from projectq_amd import Simulator, circuits
+
+    cmds, queries = circuits.qft_rotations_adder(n, 50, 15, args.seed, n_probabilities=8, prob_qubits=8, measure=False)
+    gates = [c for c in cmds if c[0] == "gate"]
+    math = [c for c in cmds if c[0] == "math"][0]
+    half = n // 2
+    ids = list(range(n))
+
+    plain = Simulator(args.seed, precision="c128", fusion="reference", capacity_qubits=n)
+    for q in ids:
+        plain.allocate_qubit(q)
+    plain.synchronize()
+    sample = gates[:60]
+    t0 = time.perf_counter()
+    for _, m, t, c in sample:
+        plain.apply_controlled_gate(m, list(t), list(c))
+        plain.run()
+    plain.synchronize()
+    plain_per_gate = (time.perf_counter() - t0) / len(sample)
+    del plain
+
+    sim = Simulator(args.seed, precision="c128", fusion="window", planner="learned", capacity_qubits=n)
+    for q in ids:
+        sim.allocate_qubit(q)
+    sim.synchronize()
+    t0 = time.perf_counter()
+    for _, m, t, c in gates:
+        sim.apply_controlled_gate(m, list(t), list(c))
+    sim.run()
+    sim.synchronize()
+    t_gates = time.perf_counter() - t0
+    plan = sim.last_plan()
+    log(f"qft+rotations+adder: {len(gates)} gates in {t_gates:.3f}s ({plan})")
+
+    rng = np.random.default_rng(args.seed + 1)
+    picks = rng.integers(0, 2**n, size=64, dtype=np.int64)
+    bits_of = lambda v: [(int(v) >> q) & 1 for q in ids]  # noqa: E731
+    before = [sim.get_amplitude(bits_of(v), ids) for v in picks]
+    t0 = time.perf_counter()
+    _, fn, regs, ctrl = math
+    sim.emulate_math(fn, regs, ctrl)
+    sim.synchronize()
+    t_add = time.perf_counter() - t0
+    mask = (1 << half) - 1
+    moved = [(v & mask) | ((((v & mask) + (v >> half)) & mask) << half) for v in picks]
+    after = [sim.get_amplitude(bits_of(v), ids) for v in moved]
+    adder_exact = all(a == b for a, b in zip(before, after))
+
+    t0 = time.perf_counter()
+    probs = [sim.get_probability(bits, qids) for bits, qids in queries]
+    t_prob = time.perf_counter() - t0
+    total = sim.get_probability([False], [0]) + sim.get_probability([True], [0])
+
+    t0 = time.perf_counter()
+    outcomes = [bool(sim.measure_qubits([q])[0]) for q in ids]
+    t_meas = time.perf_counter() - t0
+    del sim
+    t_all = t_gates + t_add + t_prob + t_meas
+    return {
+        "workload": f"{n} qubits complex128: QFT + 50 layers (random Rx/Ry/Rz per qubit + 15 controlled-Rz, "
+        f"1-3 controls) + AddQuantum(15, 15) + 8 probabilities + measure all; window fusion, learned planner",
+        "gates": len(gates),
+        "gates_s": round(t_gates, 4),
+        "gates_per_s": round(len(gates) / t_gates, 1),
+        "plan": plan,
+        "add_quantum_s": round(t_add, 4),
+        "probabilities_s": round(t_prob, 4),
+        "measure_all_s": round(t_meas, 4),
+        "total_s": round(t_all, 4),
+        "plain_c128_per_gate_ms": round(1e3 * plain_per_gate, 3),
+        "speedup_gates_vs_plain_c128": round(plain_per_gate * len(gates) / t_gates, 2),
+        "adder_moves_amplitudes_exactly": adder_exact,
+        "total_probability_minus_1": total - 1.0,
+        "probabilities": [round(p, 12) for p in probs],
+        "outcomes": "".join("1" if b else "0" for b in outcomes),
+    }
+
+
 def kernel_name(args, plan):
     """The gate-pass kernel the plan ran: the tile launcher uses the software-pipelined variant for
     complex64 tiles of 2^12 amplitudes with 4 register bits."""
@@ -245,6 +336,7 @@ def main():
     ap.add_argument("--batch", type=int, default=512, help="circuits in the batched-variational extra")
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--calibrate-copy", action="store_true", help="add a known-size D2D copy (counter scaling)")
+    ap.add_argument("--no-qft-adder", action="store_true", help="skip the 30-qubit QFT + rotations + AddQuantum extra")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -328,6 +420,9 @@ def main():
 
     if not args.no_batch:
         out["batched_variational"] = batched_variational(args)
+
+    if not args.no_qft_adder:
+        out["qft_rotations_adder_30q"] = qft_rotations_adder(args)
 
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(args)

@@ -181,6 +181,29 @@ def main():
     trec.update(pack_terms(h_terms, "h"))
     trec.update(pack_terms(op_terms, "op"))
     np.savez_compressed(os.path.join(HERE, "pauli_ops.npz"), **trec)
+
+    # 6. the QFT + rotations + AddQuantum workload at 14 qubits: QFT, 6 layers of random
+    # rotations and controlled-Rz (1-3 controls), then b <- (a + b) mod 2^7 on the two halves
+    # through the generic emulate_math callback; probabilities of 8 random 6-qubit patterns and
+    # seeded measurement outcomes of every qubit
+    n = 14
+    cmds, queries = circuits.qft_rotations_adder(n, 6, 7, 1401, n_probabilities=8, prob_qubits=6, measure=False)
+    sim = cpp.Simulator(1401)
+    commands.execute(sim, cmds, gate_fusion=False)
+    _, state = sim.cheat()
+    probs = [sim.get_probability(bits, ids) for bits, ids in queries]
+    outcomes = [sim.measure_qubits([q])[0] for q in range(n)]
+    rec = pack_gates(cmds)
+    rec.update(
+        n=np.int64(n),
+        seed=np.int64(1401),
+        state=np.asarray(state, np.complex128),
+        prob_ids=np.asarray([ids for _, ids in queries], np.int64),
+        prob_bits=np.asarray([[int(b) for b in bits] for bits, _ in queries], np.int64),
+        probs=np.asarray(probs, np.float64),
+        outcomes=np.asarray(outcomes, np.int64),
+    )
+    np.savez_compressed(os.path.join(HERE, "qft_rotations_adder14.npz"), **rec)
     print("golden fixtures written to", HERE)
 
 

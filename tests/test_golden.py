@@ -139,3 +139,23 @@ def test_time_evolution_and_qubit_operator(kind, opts):
     sim.set_wavefunction(rec["wf"], list(range(n)))
     sim.apply_qubit_operator(unpack_terms(rec, "op", real=False), list(range(n)))
     assert np.max(np.abs(sim.cheat_numpy()[1] - rec["ao_out"])) <= 1e-12
+
+
+@pytest.mark.parametrize("kind,opts", BACKENDS)
+def test_qft_rotations_adder14(kind, opts):
+    """The QFT + rotations + AddQuantum workload at 14 qubits (the 30-qubit one's shape): state
+    after b <- (a + b) mod 2^7, probabilities of 6-qubit patterns, seeded measurement of all."""
+    from projectq_amd import gates as G
+
+    rec = load("qft_rotations_adder14.npz")
+    n = int(rec["n"])
+    half = n // 2
+    sim = fresh(kind, opts, n, int(rec["seed"]))
+    cmds = unpack_gates(rec) + [("math", G.add_quantum(half), [list(range(half)), list(range(half, n))], [])]
+    commands.execute(sim, cmds, gate_fusion=bool(opts))
+    state = sim.cheat_numpy()[1]
+    assert np.max(np.abs(state - rec["state"])) <= 1e-12
+    for bits, ids, p in zip(rec["prob_bits"], rec["prob_ids"], rec["probs"]):
+        assert abs(sim.get_probability([bool(b) for b in bits], [int(q) for q in ids]) - p) <= 1e-12
+    outcomes = [int(sim.measure_qubits([q])[0]) for q in range(n)]
+    assert outcomes == [int(x) for x in rec["outcomes"]]
