@@ -439,13 +439,21 @@ __device__ __forceinline__ void reg_stage(amp_t<R> *__restrict__ tile, uint32_t 
       vr[i] = a.x;
       vi[i] = a.y;
     }
-    for (uint32_t o = ob; o < oe; ++o) {
-      const RegOp *op = rops + o;
-      const uint64_t gc = op->gcmask;
-      if ((base & gc) != gc) continue;
-      const uint32_t lc = op->lcmask;
-      if ((lb & lc) != lc) continue;
-      reg_op<R, RB>(op->code, vr, vi, mats + op->mat);
+    // Two ops per turn of the loop: the register allocator copies the amplitudes between two
+    // register sets at the loop's back edge, once per turn rather than once per op.
+    for (uint32_t o = ob; o < oe; o += 2) {
+      {
+        const RegOp *op = rops + o;
+        const uint64_t gc = op->gcmask;
+        const uint32_t lc = op->lcmask;
+        if ((base & gc) == gc && (lb & lc) == lc) reg_op<R, RB>(op->code, vr, vi, mats + op->mat);
+      }
+      if (o + 1 < oe) {
+        const RegOp *op = rops + o + 1;
+        const uint64_t gc = op->gcmask;
+        const uint32_t lc = op->lcmask;
+        if ((base & gc) == gc && (lb & lc) == lc) reg_op<R, RB>(op->code, vr, vi, mats + op->mat);
+      }
     }
     // Recompute the LDS addresses for the write-back instead of keeping 2^RB of them live across
     // the ops (the empty asm makes sb opaque to common-subexpression elimination).
@@ -646,7 +654,7 @@ __global__ __launch_bounds__(kBlock) void k_tile_pass(amp_t<R> *__restrict__ psi
 // arithmetic.  Used when a thread's share of a tile is exactly kStreamChunk 16-byte vectors
 // (2^t = kStreamChunk * kBlock * VN), so all streaming is unguarded straight-line code.
 template <typename R, int RB>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(sizeof(R) == 4 ? 4 : 1))) void k_tile_pass_pf(amp_t<R> *__restrict__ psi, uint64_t n_tiles,
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) void k_tile_pass_pf(amp_t<R> *__restrict__ psi, uint64_t n_tiles,
                                                          TilePass P, const TileStage *__restrict__ stages,
                                                          const RegOp *__restrict__ rops,
                                                          const TileOp *__restrict__ tops,
@@ -1283,10 +1291,16 @@ void launch_tile_pass(amp_t<R> *psi, uint64_t n_tiles, const TilePass &P, const 
   const uint64_t grid = std::min<uint64_t>(n_tiles, cap);
   const uint64_t chunk = uint64_t(kStreamChunk) * kBlock * Vec16<R>::n;  // amplitudes streamed per round
   const uint64_t tsize = uint64_t(1) << P.t;
-  if (prefetch && tsize == chunk && RB == 4 && sizeof(R) == 4)
-    hipLaunchKernelGGL((k_tile_pass_pf<R, RB>), dim3((unsigned)grid), dim3(kBlock), lds, s, psi, n_tiles, P,
-                       d_stages, d_rops, d_tops, d_mats);
-  else if (tsize % chunk == 0)
+  bool done = false;
+  if constexpr (RB == 4 && sizeof(R) == 4) {  // the only instantiation of the pipelined kernel
+    if (prefetch && tsize == chunk) {
+      hipLaunchKernelGGL((k_tile_pass_pf<R, RB>), dim3((unsigned)grid), dim3(kBlock), lds, s, psi, n_tiles, P,
+                         d_stages, d_rops, d_tops, d_mats);
+      done = true;
+    }
+  }
+  if (done) {
+  } else if (tsize % chunk == 0)
     hipLaunchKernelGGL((k_tile_pass<R, RB, true>), dim3((unsigned)grid), dim3(kBlock), lds, s, psi, n_tiles, P,
                        d_stages, d_rops, d_tops, d_mats);
   else
@@ -1321,9 +1335,6 @@ void prepare_tile_kernels() {
   QS_HIP(hipFuncSetAttribute((const void *)k_tile_pass<double, 4, false>, hipFuncAttributeMaxDynamicSharedMemorySize, kTileLdsMax<double>));
   QS_HIP(hipFuncSetAttribute((const void *)k_tile_pass<double, 5, false>, hipFuncAttributeMaxDynamicSharedMemorySize, kTileLdsMax<double>));
   QS_HIP(hipFuncSetAttribute((const void *)k_tile_pass_pf<float, 4>, hipFuncAttributeMaxDynamicSharedMemorySize, kTileLdsMax<float>));
-  QS_HIP(hipFuncSetAttribute((const void *)k_tile_pass_pf<float, 5>, hipFuncAttributeMaxDynamicSharedMemorySize, kTileLdsMax<float>));
-  QS_HIP(hipFuncSetAttribute((const void *)k_tile_pass_pf<double, 4>, hipFuncAttributeMaxDynamicSharedMemorySize, kTileLdsMax<double>));
-  QS_HIP(hipFuncSetAttribute((const void *)k_tile_pass_pf<double, 5>, hipFuncAttributeMaxDynamicSharedMemorySize, kTileLdsMax<double>));
   QS_HIP(hipFuncSetAttribute((const void *)k_batch_tile_pass<float, 4>, hipFuncAttributeMaxDynamicSharedMemorySize, kTileLdsMax<float>));
   QS_HIP(hipFuncSetAttribute((const void *)k_batch_tile_pass<float, 5>, hipFuncAttributeMaxDynamicSharedMemorySize, kTileLdsMax<float>));
   QS_HIP(hipFuncSetAttribute((const void *)k_batch_tile_pass<double, 4>, hipFuncAttributeMaxDynamicSharedMemorySize, kTileLdsMax<double>));
