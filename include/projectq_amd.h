@@ -145,6 +145,30 @@ int qs_run(qs_simulator *sim);
 int qs_cheat_sizes(qs_simulator *sim, size_t *n_qubits, size_t *n_amps);
 int qs_cheat(qs_simulator *sim, uint32_t *ids, uint32_t *positions, double *state);
 
+/* ---- state access without host copies, checkpoint / restore (no counterpart in the reference;
+ * they extend cheat() and set_wavefunction() for states too large for a host list) ---------- */
+
+/* Run pending gates and return the device address of the state: n_amps interleaved amplitudes,
+ * complex128 (*precision == QS_C128) or complex64 (QS_C64), index bit p <-> the qubit at position
+ * p (the map qs_cheat reports).  Valid until the next call on this simulator that applies gates,
+ * allocates, deallocates or measures (those may move the state). */
+int qs_device_state(qs_simulator *sim, void **device_ptr, uint64_t *n_amps, int *precision);
+/* The qubit map alone (no state copy): up to `capacity` (id, position) pairs, *n_qubits = all. */
+int qs_qubit_map(qs_simulator *sim, uint32_t *ids, uint32_t *positions, size_t capacity, size_t *n_qubits);
+/* Write the qubit map and the amplitudes to `path` (binary, little-endian: 32-byte header
+ * "QSAMDST1", version 1, precision, qubit count, amplitude count; then (id, position) uint32
+ * pairs; then the amplitudes in the simulator's precision), streamed through pinned memory. */
+int qs_save_state(qs_simulator *sim, const char *path);
+/* Restore a file written by qs_save_state.  A simulator without qubits allocates the file's
+ * qubits; one with qubits must hold exactly the file's qubit ids.  A file of the other precision
+ * is converted (complex128 -> complex64 rounds to nearest).  ValueError on a malformed file. */
+int qs_load_state(qs_simulator *sim, const char *path);
+/* set_wavefunction (simulator.hpp:440-454) from device memory: `device_ptr` holds n_amps
+ * interleaved amplitudes of precision `precision` on this simulator's GPU (converted if needed);
+ * ordering as in qs_set_wavefunction. */
+int qs_set_wavefunction_device(qs_simulator *sim, const void *device_ptr, uint64_t n_amps, int precision,
+                               const uint32_t *ordering, size_t n_ordering);
+
 /* ---- extensions (no counterpart in the reference) ------------------------------------------
  * Used by measurement harnesses and by tests that compare precisions on one GPU. */
 

@@ -18,6 +18,7 @@
 #include <cmath>
 #include <complex>
 #include <cstdint>
+#include <cstdio>
 #include <cstring>
 #include <map>
 #include <new>
@@ -804,6 +805,88 @@ API int qs_cheat(qs_simulator *s, uint32_t *ids, uint32_t *positions, double *st
       ++k;
     }
     std::memcpy(state, s->o.psi.data(), s->o.psi.size() * sizeof(cplx));
+  });
+}
+
+// State files (the product's qs_save_state format, restated for cross-checks): 8-byte magic
+// "QSAMDST1", uint32 version 1, uint32 precision (0 complex128, 1 complex64), uint32 qubits,
+// uint32 0, uint64 amplitudes; (id, position) uint32 pairs; amplitudes.
+API int qs_qubit_map(qs_simulator *s, uint32_t *ids, uint32_t *positions, size_t cap, size_t *n) {
+  return guarded([&] {
+    s->o.run();
+    size_t k = 0;
+    for (auto &kv : s->o.pos) {
+      if (k < cap) {
+        ids[k] = kv.first;
+        positions[k] = kv.second;
+      }
+      ++k;
+    }
+    *n = k;
+  });
+}
+API int qs_device_state(qs_simulator *, void **, uint64_t *, int *) {
+  return guarded([&] { throw std::runtime_error("the oracle keeps its state in host memory"); });
+}
+API int qs_set_wavefunction_device(qs_simulator *, const void *, uint64_t, int, const uint32_t *, size_t) {
+  return guarded([&] { throw std::runtime_error("the oracle keeps its state in host memory"); });
+}
+API int qs_save_state(qs_simulator *s, const char *path) {
+  return guarded([&] {
+    s->o.run();
+    FILE *f = std::fopen(path, "wb");
+    if (!f) throw std::runtime_error("cannot open state file");
+    const uint32_t nq = (uint32_t)s->o.pos.size();
+    const uint64_t na = s->o.psi.size();
+    const uint32_t head[4] = {1u, 0u, nq, 0u};
+    bool ok = std::fwrite("QSAMDST1", 1, 8, f) == 8 && std::fwrite(head, 4, 4, f) == 4 && std::fwrite(&na, 8, 1, f) == 1;
+    for (auto &kv : s->o.pos) {
+      const uint32_t rec[2] = {kv.first, kv.second};
+      ok = ok && std::fwrite(rec, 4, 2, f) == 2;
+    }
+    ok = ok && std::fwrite(s->o.psi.data(), sizeof(cplx), na, f) == na;
+    std::fclose(f);
+    if (!ok) throw std::runtime_error("save_state: write failed");
+  });
+}
+API int qs_load_state(qs_simulator *s, const char *path) {
+  return guarded([&] {
+    s->o.run();
+    FILE *f = std::fopen(path, "rb");
+    if (!f) throw std::runtime_error("cannot open state file");
+    char magic[8];
+    uint32_t head[4];
+    uint64_t na = 0;
+    bool ok = std::fread(magic, 1, 8, f) == 8 && std::memcmp(magic, "QSAMDST1", 8) == 0 &&
+              std::fread(head, 4, 4, f) == 4 && std::fread(&na, 8, 1, f) == 1 && head[0] == 1 && head[1] <= 1 &&
+              head[2] <= 30 && na == (uint64_t(1) << head[2]);
+    std::map<unsigned, unsigned> map;
+    for (uint32_t i = 0; ok && i < head[2]; ++i) {
+      uint32_t rec[2];
+      ok = std::fread(rec, 4, 2, f) == 2 && rec[1] < head[2] && !map.count(rec[0]);
+      if (ok) map[rec[0]] = rec[1];
+    }
+    std::vector<cplx> amps(ok ? na : 0);
+    if (ok && head[1] == 0) {
+      ok = std::fread(amps.data(), sizeof(cplx), na, f) == na;
+    } else if (ok) {
+      std::vector<float> v(2 * na);
+      ok = std::fread(v.data(), 8, na, f) == na;
+      for (uint64_t i = 0; ok && i < na; ++i) amps[i] = cplx(v[2 * i], v[2 * i + 1]);
+    }
+    std::fclose(f);
+    if (!ok) throw std::invalid_argument("load_state: malformed state file");
+    if (s->o.pos.empty()) {
+      std::vector<unsigned> by_pos(head[2]);
+      for (auto &kv : map) by_pos[kv.second] = kv.first;
+      for (unsigned id : by_pos) s->o.allocate(id);
+    } else {
+      bool same = s->o.pos.size() == map.size();
+      for (auto &kv : map) same = same && s->o.pos.count(kv.first);
+      if (!same) throw std::runtime_error("load_state: the simulator holds other qubits than the file");
+    }
+    s->o.pos = map;
+    s->o.psi = amps;
   });
 }
 

@@ -10,6 +10,7 @@ the HIP library; nothing in the package binds anything else.
 """
 
 import ctypes
+import os
 import math
 
 import numpy as np
@@ -171,6 +172,17 @@ _SIGNATURES = {
         [ctypes.c_void_p, _u32p, _u32p, _u32p, _u32p],
     ),
     "qs_batch_destroy": (None, [ctypes.c_void_p]),
+    "qs_device_state": (
+        ctypes.c_int,
+        [ctypes.c_void_p, ctypes.POINTER(ctypes.c_void_p), ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_int)],
+    ),
+    "qs_qubit_map": (ctypes.c_int, [ctypes.c_void_p, _u32p, _u32p, ctypes.c_size_t, _szp]),
+    "qs_save_state": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p]),
+    "qs_load_state": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p]),
+    "qs_set_wavefunction_device": (
+        ctypes.c_int,
+        [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint64, ctypes.c_int, _u32p, ctypes.c_size_t],
+    ),
 }
 
 PASS_FEATURES = 24
@@ -597,10 +609,91 @@ class CAbiSimulator:
         mapping = {int(ids[i]): int(pos[i]) for i in range(nq.value)}
         return mapping, state
 
+    def qubit_map(self):
+        """dict qubit id -> bit position, without copying the state."""
+        n = ctypes.c_size_t()
+        self._check(self._lib.qs_qubit_map(self._handle, None, None, ctypes.c_size_t(0), ctypes.byref(n)))
+        ids = np.zeros(max(n.value, 1), dtype=np.uint32)
+        pos = np.zeros(max(n.value, 1), dtype=np.uint32)
+        self._check(
+            self._lib.qs_qubit_map(
+                self._handle, _ptr(ids, ctypes.c_uint32), _ptr(pos, ctypes.c_uint32), ctypes.c_size_t(n.value), ctypes.byref(n)
+            )
+        )
+        return {int(ids[i]): int(pos[i]) for i in range(n.value)}
+
+    def device_state(self):
+        """The state in device memory without a host copy: (dict id -> position, DeviceState).
+
+        The DeviceState aliases the simulator's buffer (complex64 or complex128, index bit p <->
+        the qubit at position p) and is valid until the next call that applies gates, allocates,
+        deallocates or measures.  It exposes `__cuda_array_interface__`, so array libraries that
+        share this process's HIP runtime can wrap it without a copy (import torch before
+        projectq_amd: torch's libraries load their own HIP runtime otherwise)."""
+        ptr = ctypes.c_void_p()
+        n = ctypes.c_uint64()
+        prec = ctypes.c_int()
+        self._check(self._lib.qs_device_state(self._handle, ctypes.byref(ptr), ctypes.byref(n), ctypes.byref(prec)))
+        return self.qubit_map(), DeviceState(int(ptr.value or 0), int(n.value), prec.value == 1, self)
+
+    def save_state(self, path):
+        """Checkpoint the qubit map and the state to a binary file (qs_save_state)."""
+        self._check(self._lib.qs_save_state(self._handle, os.fsencode(path)))
+
+    def load_state(self, path):
+        """Restore a checkpoint written by save_state (qs_load_state)."""
+        self._check(self._lib.qs_load_state(self._handle, os.fsencode(path)))
+
+    def set_wavefunction_device(self, src, ordering):
+        """set_wavefunction from device memory on this GPU: a DeviceState, or any object with
+        `__cuda_array_interface__` (a 1-d contiguous complex64 or complex128 array, e.g. a torch
+        tensor), converted to this simulator's precision if needed."""
+        iface = src.__cuda_array_interface__
+        shape, typestr, strides = iface["shape"], iface["typestr"], iface.get("strides")
+        if len(shape) != 1 or strides not in (None, (np.dtype(typestr).itemsize,)):
+            raise ValueError("set_wavefunction_device: need a contiguous 1-d array")
+        if typestr == "<c8":
+            prec = 1
+        elif typestr == "<c16":
+            prec = 0
+        else:
+            raise ValueError("set_wavefunction_device: complex64 or complex128 amplitudes")
+        stream = iface.get("stream")
+        if hasattr(src, "is_cuda") and stream is None:  # torch: finish its pending work first
+            import torch
+
+            torch.cuda.current_stream().synchronize()
+        order = np.asarray(list(ordering), dtype=np.uint32)
+        self._check(
+            self._lib.qs_set_wavefunction_device(
+                self._handle, ctypes.c_void_p(int(iface["data"][0])), ctypes.c_uint64(int(shape[0])), ctypes.c_int(prec),
+                _ptr(order, ctypes.c_uint32), ctypes.c_size_t(len(order)),
+            )
+        )
+
     def cheat(self):
         """simulator.hpp:529-532: (dict id -> bit position, list of complex amplitudes)."""
         mapping, state = self.cheat_numpy()
         return mapping, state.tolist()
+
+
+class DeviceState:
+    """A simulator's state in device memory, without a copy (see CAbiSimulator.device_state).
+    `ptr` is the device address, `n_amps` the amplitude count, `c64` the precision."""
+
+    def __init__(self, ptr, n_amps, c64, owner):
+        self.ptr, self.n_amps, self.c64 = ptr, n_amps, c64
+        self._owner = owner  # keeps the simulator (and so the buffer) alive
+
+    @property
+    def __cuda_array_interface__(self):
+        return {
+            "shape": (self.n_amps,),
+            "typestr": "<c8" if self.c64 else "<c16",
+            "data": (self.ptr, False),
+            "version": 2,
+            "strides": None,
+        }
 
 
 class CAbiBatch:

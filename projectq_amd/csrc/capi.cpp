@@ -207,6 +207,33 @@ API int qs_cheat(qs_simulator *s, uint32_t *ids, uint32_t *positions, double *st
   return guarded(s, [&] { s->engine.cheat(ids, positions, state); });
 }
 
+API int qs_device_state(qs_simulator *s, void **device_ptr, uint64_t *n_amps, int *precision) {
+  return guarded(s, [&] { s->engine.device_state(device_ptr, n_amps, precision); });
+}
+API int qs_qubit_map(qs_simulator *s, uint32_t *ids, uint32_t *positions, size_t capacity, size_t *n_qubits) {
+  return guarded(s, [&] { s->engine.qubit_map(ids, positions, capacity, n_qubits); });
+}
+API int qs_save_state(qs_simulator *s, const char *path) {
+  return guarded(s, [&] {
+    if (!path) throw qs::ValueError("save_state: no path");
+    s->engine.save_state(path);
+  });
+}
+API int qs_load_state(qs_simulator *s, const char *path) {
+  return guarded(s, [&] {
+    if (!path) throw qs::ValueError("load_state: no path");
+    s->engine.load_state(path);
+  });
+}
+API int qs_set_wavefunction_device(qs_simulator *s, const void *device_ptr, uint64_t n_amps, int precision,
+                                   const uint32_t *ordering, size_t n_ordering) {
+  return guarded(s, [&] {
+    if (!device_ptr && n_amps) throw qs::ValueError("set_wavefunction_device: null device pointer");
+    std::vector<unsigned> ord(ordering, ordering + n_ordering);
+    s->engine.set_wavefunction_device(device_ptr, n_amps, precision, ord);
+  });
+}
+
 API int qs_synchronize(qs_simulator *s) {
   return guarded(s, [&] { s->engine.synchronize(); });
 }

@@ -1018,4 +1018,207 @@ void Engine::cheat(uint32_t *ids, uint32_t *positions, double *state) {
   }
 }
 
+
+// The state stays where it is: the pointer is valid until the next call on this simulator that
+// applies gates, allocates or frees qubits (those may move the state to the other buffer).
+void Engine::device_state(void **ptr, uint64_t *n_amps, int *precision) {
+  run();
+  sync();
+  *ptr = psi_;
+  *n_amps = amps();
+  *precision = c64_ ? 1 : 0;
+}
+
+void Engine::qubit_map(uint32_t *ids, uint32_t *positions, size_t capacity, size_t *n_qubits) {
+  run();
+  size_t k = 0;
+  for (auto &kv : pos_) {
+    if (k < capacity) {
+      ids[k] = kv.first;
+      positions[k] = kv.second;
+    }
+    ++k;
+  }
+  *n_qubits = k;
+}
+
+namespace {
+// Checkpoint file: header, the qubit map, then the amplitudes (interleaved re/im, little-endian)
+// in the simulator's precision.
+constexpr char kStateMagic[8] = {'Q', 'S', 'A', 'M', 'D', 'S', 'T', '1'};
+struct StateHeader {
+  char magic[8];
+  uint32_t version;    // 1
+  uint32_t precision;  // 0 complex128, 1 complex64
+  uint32_t n_qubits;
+  uint32_t reserved;
+  uint64_t n_amps;
+};
+constexpr size_t kStageBytes = size_t(64) << 20;  // pinned staging buffer (two of them)
+
+struct PinnedPair {
+  void *p[2] = {nullptr, nullptr};
+  PinnedPair() {
+    for (auto &b : p) QS_HIP(hipHostMalloc(&b, kStageBytes, hipHostMallocDefault));
+  }
+  ~PinnedPair() {
+    for (auto &b : p)
+      if (b) (void)hipHostFree(b);
+  }
+};
+struct File {
+  FILE *f;
+  File(const char *path, const char *mode) : f(std::fopen(path, mode)) {
+    if (!f) throw RuntimeError(std::string("cannot open state file ") + path);
+  }
+  ~File() {
+    if (f) std::fclose(f);
+  }
+};
+}  // namespace
+
+void Engine::save_state(const char *path) {
+  run();
+  sync();
+  File file(path, "wb");
+  StateHeader h{};
+  std::memcpy(h.magic, kStateMagic, 8);
+  h.version = 1;
+  h.precision = c64_ ? 1 : 0;
+  h.n_qubits = n_;
+  h.n_amps = amps();
+  if (std::fwrite(&h, sizeof h, 1, file.f) != 1) throw RuntimeError("save_state: write failed");
+  for (auto &kv : pos_) {
+    const uint32_t rec[2] = {kv.first, kv.second};
+    if (std::fwrite(rec, sizeof rec, 1, file.f) != 1) throw RuntimeError("save_state: write failed");
+  }
+  // device -> pinned (async, alternating buffers) -> file
+  PinnedPair stage;
+  const uint64_t total = amps() * amp_bytes();
+  const char *src = static_cast<const char *>(psi_);
+  hipEvent_t done[2];
+  for (auto &e : done) QS_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+  uint64_t off = 0;
+  int k = 0;
+  uint64_t pending = 0;  // bytes in flight into stage.p[k ^ 1]
+  try {
+    while (off < total || pending) {
+      uint64_t len = 0;
+      if (off < total) {
+        len = std::min<uint64_t>(kStageBytes, total - off);
+        QS_HIP(hipMemcpyAsync(stage.p[k], src + off, len, hipMemcpyDeviceToHost, stream_));
+        QS_HIP(hipEventRecord(done[k], stream_));
+        off += len;
+      }
+      if (pending) {
+        QS_HIP(hipEventSynchronize(done[k ^ 1]));
+        if (std::fwrite(stage.p[k ^ 1], 1, pending, file.f) != pending) throw RuntimeError("save_state: write failed");
+      }
+      pending = len;
+      k ^= 1;
+    }
+  } catch (...) {
+    (void)hipStreamSynchronize(stream_);
+    for (auto &e : done) (void)hipEventDestroy(e);
+    throw;
+  }
+  for (auto &e : done) QS_HIP(hipEventDestroy(e));
+}
+
+void Engine::load_state(const char *path) {
+  run();
+  File file(path, "rb");
+  StateHeader h{};
+  if (std::fread(&h, sizeof h, 1, file.f) != 1 || std::memcmp(h.magic, kStateMagic, 8) != 0 || h.version != 1)
+    throw ValueError("load_state: not a state file of this simulator");
+  if (h.n_qubits > 40 || h.n_amps != (uint64_t(1) << h.n_qubits) || h.precision > 1)
+    throw ValueError("load_state: corrupt header");
+  std::map<unsigned, unsigned> map;
+  std::vector<bool> seen(h.n_qubits, false);
+  for (uint32_t i = 0; i < h.n_qubits; ++i) {
+    uint32_t rec[2];
+    if (std::fread(rec, sizeof rec, 1, file.f) != 1) throw ValueError("load_state: truncated qubit map");
+    if (rec[1] >= h.n_qubits || seen[rec[1]] || map.count(rec[0])) throw ValueError("load_state: corrupt qubit map");
+    seen[rec[1]] = true;
+    map[rec[0]] = rec[1];
+  }
+  if (!pos_.empty()) {
+    bool same = pos_.size() == map.size();
+    for (auto &kv : map) same = same && pos_.count(kv.first);
+    if (!same) throw RuntimeError("load_state: the simulator holds other qubits than the file");
+  } else {
+    std::vector<unsigned> by_pos(h.n_qubits);
+    for (auto &kv : map) by_pos[kv.second] = kv.first;
+    for (unsigned id : by_pos) allocate_qubit(id);
+  }
+  pos_ = map;
+  const bool file_c64 = h.precision == 1;
+  const size_t file_amp = file_c64 ? 8 : 16;
+  const uint64_t total = h.n_amps * file_amp;
+  PinnedPair stage;
+  char *dst = static_cast<char *>(psi_);
+  uint64_t off = 0;
+  int k = 0;
+  sync();
+  while (off < total) {
+    const uint64_t len = std::min<uint64_t>(kStageBytes - kStageBytes % 16, total - off);
+    // the previous copy out of stage.p[k] must be done before it is refilled
+    QS_HIP(hipStreamSynchronize(stream_));
+    if (std::fread(stage.p[k], 1, len, file.f) != len) throw ValueError("load_state: truncated amplitudes");
+    const uint64_t first_amp = off / file_amp, n = len / file_amp;
+    if (file_c64 == c64_) {
+      QS_HIP(hipMemcpyAsync(dst + off, stage.p[k], len, hipMemcpyHostToDevice, stream_));
+    } else {
+      // convert precision in place on the host side of the other buffer, then copy
+      void *other = stage.p[k ^ 1];
+      if (c64_) {
+        const double *in = static_cast<const double *>(stage.p[k]);
+        float *out = static_cast<float *>(other);
+        for (uint64_t i = 0; i < 2 * n; ++i) out[i] = (float)in[i];
+        QS_HIP(hipMemcpyAsync(dst + first_amp * 8, other, n * 8, hipMemcpyHostToDevice, stream_));
+      } else {
+        // complex64 file into a complex128 simulator: two halves so the output fits the buffer
+        const float *in = static_cast<const float *>(stage.p[k]);
+        double *out = static_cast<double *>(other);
+        const uint64_t half = n / 2;
+        for (uint64_t part = 0; part < 2; ++part) {
+          const uint64_t a = part ? half : 0, b = part ? n : half;
+          for (uint64_t i = 2 * a; i < 2 * b; ++i) out[i - 2 * a] = (double)in[i];
+          QS_HIP(hipMemcpyAsync(dst + (first_amp + a) * 16, other, (b - a) * 16, hipMemcpyHostToDevice, stream_));
+          QS_HIP(hipStreamSynchronize(stream_));
+        }
+      }
+    }
+    off += len;
+    k ^= 1;
+  }
+  sync();
+}
+
+void Engine::set_wavefunction_device(const void *src, uint64_t n_amps, int src_precision,
+                                     const std::vector<unsigned> &ordering) {
+  run();
+  bool ok = pos_.size() == ordering.size();
+  for (unsigned q : ordering) ok = ok && pos_.count(q);
+  if (!ok)
+    throw RuntimeError(
+        "set_wavefunction(): Invalid mapping provided. Please make sure all qubits have been allocated "
+        "previously (call eng.flush()).");
+  if (n_amps != (uint64_t(1) << ordering.size()))
+    throw ValueError("set_wavefunction(): the wavefunction must hold 2^len(ordering) amplitudes");
+  if (src_precision != 0 && src_precision != 1) throw ValueError("set_wavefunction_device: unknown precision");
+  for (size_t i = 0; i < ordering.size(); ++i) pos_[ordering[i]] = (unsigned)i;
+  const bool src_c64 = src_precision == 1;
+  if (src_c64 == c64_) {
+    QS_HIP(hipMemcpyAsync(psi_, src, n_amps * amp_bytes(), hipMemcpyDeviceToDevice, stream_));
+  } else if (c64_) {
+    dev::convert_amps<double, float>(static_cast<const amp_t<double> *>(src), static_cast<amp_t<float> *>(psi_),
+                                     n_amps, stream_);
+  } else {
+    dev::convert_amps<float, double>(static_cast<const amp_t<float> *>(src), static_cast<amp_t<double> *>(psi_),
+                                     n_amps, stream_);
+  }
+  sync();
+}
+
 }  // namespace qs

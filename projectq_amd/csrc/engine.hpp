@@ -59,6 +59,14 @@ class Engine {
   void run();
   void cheat_sizes(size_t *n_qubits, size_t *n_amps);
   void cheat(uint32_t *ids, uint32_t *positions, double *state);
+  // Extensions: the state in device memory without a host copy, checkpoint/restore to a file,
+  // and set_wavefunction from device memory.
+  void device_state(void **ptr, uint64_t *n_amps, int *precision);
+  void qubit_map(uint32_t *ids, uint32_t *positions, size_t capacity, size_t *n_qubits);
+  void save_state(const char *path);
+  void load_state(const char *path);
+  void set_wavefunction_device(const void *src, uint64_t n_amps, int src_precision,
+                               const std::vector<unsigned> &ordering);
 
   // execute one fused block (also used by the planners)
   void exec_block(const Block &b);

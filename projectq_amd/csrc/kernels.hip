@@ -1526,6 +1526,28 @@ void math_accumulate(const amp_t<R> *src, amp_t<R> *dst, uint64_t n, const MathA
 }
 
 // ------------------------------------------------------------------------------ instantiations
+template <typename S, typename D>
+__global__ __launch_bounds__(kBlock) void k_convert_amps(const amp_t<S> *__restrict__ src, amp_t<D> *__restrict__ dst,
+                                                         uint64_t n) {
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+    const amp_t<S> a = src[i];
+    amp_t<D> b;
+    b.x = (D)a.x;
+    b.y = (D)a.y;
+    dst[i] = b;
+  }
+}
+
+template <typename S, typename D>
+void convert_amps(const amp_t<S> *src, amp_t<D> *dst, uint64_t n, hipStream_t s) {
+  if (n == 0) return;
+  hipLaunchKernelGGL((k_convert_amps<S, D>), dim3(grid_for(n)), dim3(kBlock), 0, s, src, dst, n);
+  QS_HIP(hipGetLastError());
+}
+template void convert_amps<double, float>(const amp_t<double> *, amp_t<float> *, uint64_t, hipStream_t);
+template void convert_amps<float, double>(const amp_t<float> *, amp_t<double> *, uint64_t, hipStream_t);
+
 #define QS_INSTANTIATE(R)                                                                          \
   template void batch_init<R>(amp_t<R> *, uint64_t, const uint64_t *, int, hipStream_t);          \
   template void batch_tile_pass<R>(amp_t<R> *, const BatchJob *, const uint32_t *, uint32_t,       \

@@ -134,5 +134,9 @@ void math_accumulate(const amp_t<R> *src, amp_t<R> *dst, uint64_t n_amps, const 
 void overlap(const void *a, bool a_c64, const void *b, bool b_c64, uint64_t n_amps,
              double *partials, hipStream_t s);
 
+// dst[i] = src[i] converted between complex128 and complex64 (round to nearest on narrowing)
+template <typename S, typename D>
+void convert_amps(const amp_t<S> *src, amp_t<D> *dst, uint64_t n, hipStream_t s);
+
 }  // namespace dev
 }  // namespace qs
