@@ -1,0 +1,160 @@
+"""The default compiler engine list (projectq_amd.compiler) against call sequences recorded from
+ProjectQ's own MainEngine (tests/golden/make_engine_traces.py writes engine_traces.json).
+
+Each recorded program is replayed through `projectq_amd.compiler.run_program` in front of the
+same call recorder; the calls must agree one for one: the same methods in the same order, the same
+qubit ids and controls, bit-identical gate matrices and identical math tables."""
+
+import contextlib
+import io
+import itertools
+import json
+import math
+import os
+
+import numpy as np
+import pytest
+
+from projectq_amd import compiler as C
+from trace_recorder import Recorder
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+with open(os.path.join(HERE, "golden", "engine_traces.json")) as _f:
+    TRACES = json.load(_f)["programs"]
+
+
+def replay(ops):
+    rec = Recorder()
+    backend = C.SimulatorEngine(rec)
+    error = None
+    out = io.StringIO()
+    with contextlib.redirect_stdout(out):
+        try:
+            C.run_program(ops, backend)
+        except Exception as err:  # noqa: BLE001 -- compared with the reference's error
+            error = type(err).__name__
+    return rec.calls, error, out.getvalue()
+
+
+def first_difference(got, want):
+    for k, (a, b) in enumerate(zip(got, want)):
+        if a != b:
+            return k, a, b
+    return min(len(got), len(want)), None, None
+
+
+@pytest.mark.parametrize("name", sorted(TRACES))
+def test_call_sequence_matches_reference(name):
+    prog = TRACES[name]
+    got, error, _ = replay(prog["ops"])
+    assert error == prog["error"]
+    if got != prog["calls"]:
+        k, a, b = first_difference(got, prog["calls"])
+        pytest.fail(f"{name}: call {k} differs (got {len(got)} calls, reference {len(prog['calls'])}):\n"
+                    f"  got       {a}\n  reference {b}")
+
+
+def test_identity_on_several_qubits_reports_a_broken_pipeline():
+    """An identity command on several qubits is removed from its last qubit's pipeline only; the
+    reference then prints its broken-pipeline message (and the call sequence above still agrees)."""
+    _, error, printed = replay(TRACES["identity_multiqubit"]["ops"])
+    assert error is None
+    assert "Invalid qubit pipeline encountered" in printed
+
+
+def test_rotation_algebra():
+    """Closed forms of the angle rules (projectq/ops/_basics.py:321-502)."""
+    assert C.Rx(4 * math.pi - 1e-13).angle == 0.0 and C.Rx(4 * math.pi - 1e-13).is_identity()
+    assert C.Rz(-math.pi).angle == round(3 * math.pi, 12)
+    assert C.Rz(1.0).get_inverse() == C.Rz(4 * math.pi - 1.0)
+    assert C.Rz(0).get_inverse() == C.Rz(0)
+    assert C.Rz(1.0).get_merged(C.Rz(2.0)) == C.Rz(3.0)
+    with pytest.raises(C.NotMergeable):
+        C.Rz(1.0).get_merged(C.Rx(1.0))
+    assert C.R(2 * math.pi).angle == 0.0 and not C.R(0.0).is_identity()  # phase gates never are
+    assert C.Ph(1.0).get_inverse() == C.Ph(2 * math.pi - 1.0)
+    assert isinstance(C.get_inverse(C.S), C.DaggeredGate) and C.get_inverse(C.get_inverse(C.S)) == C.S
+    assert np.array_equal(C.Sdag.matrix, C.S.matrix.conj().T)
+    # a matrix gate equals any gate with a close matrix, in either order
+    assert C.MatrixGate([[0, 1], [1, 0]]) == C.X and C.X == C.MatrixGate([[0, 1], [1, 0]])
+    assert C.MatrixGate([[0, 1], [1, 0]]) != C.Y
+
+
+def test_command_canonical_forms():
+    swap_ab = C.Command(C.Swap, [[3], [1]])
+    assert swap_ab.qubits == ([1], [3]) and swap_ab == C.Command(C.Swap, [[1], [3]])
+    assert C.Command(C.Rzz(0.1), [[3], [1]]).qubits == ([3], [1])  # not interchangeable
+    cmd = C.Command(C.X, [[0]])
+    cmd.add_control_qubits([5, 2], "01")
+    assert cmd.control_qubits == [2, 5] and cmd.control_state == "10"
+    with pytest.raises(C.IncompatibleControlState):
+        cmd.add_control_qubits([2], "0")
+    assert C.Command(C.X, [[0]], [1], control_state="0").get_inverse().control_state == "1"
+    assert C.canonical_ctrl_state(6, 3) == "011"
+
+
+def test_qft_expansion_order():
+    """qft2crandhadamard.py:31-39: H on q[-1-i], then R(pi/2^(j+1)) on it controlled by q[-1-(j+i+1)]."""
+    rec = Recorder()
+    C.run_program([["alloc", 3], ["gate", "QFT", [], [[0, 1, 2]], [], None]], C.SimulatorEngine(rec),
+                  engine_list=[C.AutoReplacer()])
+    gates = [(c[2], c[3]) for c in rec.calls if c[0] == "apply_controlled_gate"]
+    assert gates == [([2], []), ([2], [1]), ([2], [0]), ([1], []), ([1], [0]), ([0], [])]
+
+
+@pytest.mark.parametrize("name", ["layers6", "qft", "math", "compute_uncompute", "random3"])
+def test_default_list_runs_on_the_oracle_like_the_recorded_calls(name):
+    """The compiled program and the reference's recorded calls, each run on the oracle with the
+    same seed, measure the same outcomes and end in the same state (one amplitude once every
+    qubit is deallocated: the accumulated global phase)."""
+    from oracle.cpu_sim import OracleSimulator
+
+    a = OracleSimulator(7)
+    eng = C.run_program(TRACES[name]["ops"], C.SimulatorEngine(a))
+    b = OracleSimulator(7)
+    outcomes = {}
+    for call in TRACES[name]["calls"]:
+        out = play_call(b, call)
+        if call[0] == "measure_qubits":
+            outcomes.update(zip(call[1], out))
+    assert eng.results == outcomes
+    assert a.cheat_numpy()[0] == b.cheat_numpy()[0] == {}
+    # equal calls; the oracle's threaded norm sums may round differently from run to run
+    assert abs(a.cheat_numpy()[1][0] - b.cheat_numpy()[1][0]) <= 1e-12
+
+
+def play_call(sim, call):
+    kind = call[0]
+    if kind == "apply_controlled_gate":
+        m = np.array([[complex(re, im) for re, im in row] for row in call[1]])
+        return sim.apply_controlled_gate(m, call[2], call[3])
+    if kind == "emulate_math":
+        table, regs = call[1], call[2]
+        lookup = {}
+        for values, res in zip(itertools.product(*[range(1 << len(r)) for r in regs]), table):
+            lookup[tuple(values)] = res
+        return sim.emulate_math(lambda v: list(lookup[tuple(v)]), regs, call[3])
+    return getattr(sim, kind)(*call[1:])
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("precision", ["c128", "c64"])
+def test_default_list_on_the_gpu(precision):
+    """A recorded program compiled by the default list and run on the HIP simulator, against the
+    oracle replaying the reference's recorded calls."""
+    from oracle.cpu_sim import OracleSimulator
+    from projectq_amd import Simulator
+
+    for name in ("layers6", "qft", "math", "random5"):
+        ops = [op for op in TRACES[name]["ops"] if op[0] not in ("dealloc", "measure")]
+        calls = [c for c in TRACES[name]["calls"] if c[0] not in ("deallocate_qubit", "measure_qubits")]
+        h = Simulator(3, precision=precision)
+        C.run_program(ops, C.SimulatorEngine(h))
+        o = OracleSimulator(3)
+        for call in calls:
+            play_call(o, call)
+        got_map, got = h.cheat_numpy()
+        want_map, want = o.cheat_numpy()
+        assert got_map == want_map
+        tol = 1e-12 if precision == "c128" else 2e-6
+        assert np.max(np.abs(got - want)) <= tol, name
