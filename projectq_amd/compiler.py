@@ -28,6 +28,7 @@ Restated pieces and the reference code each follows:
   (setups/decompositions/controlstate.py) and barriers (setups/decompositions/barrier.py);
 * the meta sections Control, Compute/Uncompute (without qubit allocation inside) and Dagger
   (projectq/meta/_control.py, _compute.py, _dagger.py);
+* `TimeEvolution` with the index renumbering of its `|` operator (projectq/ops/_time_evolution.py);
 * `SimulatorEngine` -- the backend's `is_available`, `_handle` and `receive`.
 
 The optimizer's list operations are kept step for step, including two behaviours of the reference
@@ -418,6 +419,66 @@ class ComparatorQuantumGate(BasicMathGate):
 
     def math_function(self, quregs):
         return G.comparator_quantum()
+
+
+class NotHermitianOperatorError(ValueError):
+    pass
+
+
+class TimeEvolution(BasicGate):
+    """exp(-i t H) for a Pauli-sum H with real coefficients (projectq/ops/_time_evolution.py).
+
+    `hamiltonian` is a dict or a list of (term, coefficient) pairs, a term being a tuple of
+    (qubit index, 'X' | 'Y' | 'Z') sorted by index; the order of the pairs is the order the
+    backend receives the terms in.  Two TimeEvolution gates compare equal only when they are the
+    same object (the reference's __eq__ returns NotImplemented), so the optimizer never merges or
+    cancels them."""
+
+    name = "TimeEvolution"
+
+    def __init__(self, time, hamiltonian):
+        if not isinstance(time, (float, int)):
+            raise TypeError("time needs to be a (real) numeric type.")
+        items = hamiltonian.items() if isinstance(hamiltonian, dict) else hamiltonian
+        self.time = time
+        self.terms = {}
+        for term, coeff in items:
+            term = tuple(sorted(((int(q), str(p)) for q, p in term), key=lambda lp: lp[0]))
+            coeff = complex(coeff)
+            if coeff.imag != 0:
+                raise NotHermitianOperatorError("hamiltonian must be hermitian and hence only have real coefficients.")
+            self.terms[term] = float(coeff.real)
+
+    def get_inverse(self):
+        return TimeEvolution(self.time * -1.0, self.terms)
+
+    def get_merged(self, other):
+        if isinstance(other, TimeEvolution) and set(self.terms) == set(other.terms):
+            factor = None
+            for term in self.terms:
+                ratio = self.terms[term] / float(other.terms[term])
+                if factor is None:
+                    factor = ratio
+                elif not abs(factor - ratio) <= 1e-9 * max(abs(factor), abs(ratio)):
+                    raise NotMergeable("Cannot merge these two gates.")
+            return TimeEvolution(self.time + other.time / factor, self.terms)
+        raise NotMergeable("Cannot merge these two gates.")
+
+    def __eq__(self, other):
+        return NotImplemented
+
+    def commands_for(self, qureg):
+        """What `TimeEvolution(...) | qureg` emits (_time_evolution.py __or__): a phase gate on
+        the first qubit for an identity-only H, else the gate on the qubits H acts on, with
+        the term indices renumbered to positions in that list."""
+        if len(self.terms) == 1 and () in self.terms:
+            return Ph(-1 * self.time * self.terms[()]), [qureg[0]]
+        used = sorted({q for term in self.terms for q, _ in term})
+        if used[-1] >= len(qureg):
+            raise ValueError("hamiltonian acts on more qubits than the gate is applied to.")
+        pos = {q: i for i, q in enumerate(used)}
+        terms = [(tuple((pos[q], p) for q, p in term), c) for term, c in self.terms.items()]
+        return TimeEvolution(self.time, terms), [qureg[q] for q in used]
 
 
 # shortcuts, as projectq.ops / projectq.libs.math export them
@@ -831,6 +892,11 @@ class _Frontend:
         return stage
 
     def apply(self, gate, *regs, controls=(), control_state=None):
+        if isinstance(gate, TimeEvolution):
+            if len(regs) != 1:
+                raise TypeError("Only one qubit or qureg allowed.")
+            gate, reg = gate.commands_for(list(regs[0]))
+            regs = (reg,)
         cmd = Command(gate, [list(r) for r in regs])
         if len(controls):
             cmd.add_control_qubits(list(controls), control_state)
@@ -898,7 +964,7 @@ class SimulatorEngine(Engine):
     def is_available(self, cmd):
         if has_negative_control(cmd):
             return False
-        if isinstance(cmd.gate, (MeasureGate, AllocateQubitGate, DeallocateQubitGate, BasicMathGate)):
+        if isinstance(cmd.gate, (MeasureGate, AllocateQubitGate, DeallocateQubitGate, BasicMathGate, TimeEvolution)):
             return True
         m = cmd.gate.matrix
         return m is not None and len(m) <= 2**5
@@ -935,6 +1001,9 @@ class SimulatorEngine(Engine):
                 self.sim.emulate_math_multiplyByConstantModN(gate.a, gate.N, quregs, ctrl)
             else:
                 self.sim.emulate_math(gate.math_function(quregs), quregs, ctrl)
+        elif isinstance(gate, TimeEvolution):
+            terms = [(list(term), coeff) for term, coeff in gate.terms.items()]
+            self.sim.emulate_time_evolution(terms, gate.time, list(cmd.qubits[0]), ctrl)
         elif gate.matrix is not None and len(gate.matrix) <= 2**5:
             m = gate.matrix
             ids = [q for reg in cmd.qubits for q in reg]
@@ -1012,6 +1081,8 @@ def make_gate(name, params=()):
         return get_inverse(_FIXED_GATES[name[:-3]]())
     if name == "Matrix":
         return MatrixGate([[complex(re, im) for re, im in row] for row in params])
+    if name == "TimeEvolution":  # params = [time, [[term, coeff], ...]], term = [[index, "X"], ...]
+        return TimeEvolution(params[0], [(tuple(tuple(lp) for lp in term), c) for term, c in params[1]])
     if name in _PARAM_GATES:
         return _PARAM_GATES[name](*params)
     raise ValueError(f"unknown gate {name!r}")

@@ -170,6 +170,26 @@ def prog_identity_multiqubit():
     return ops
 
 
+def te(time, terms):
+    """TimeEvolution params: terms as [[[index, "X"], ...], coefficient] pairs."""
+    return [time, [[[list(lp) for lp in term], c] for term, c in terms]]
+
+
+def prog_time_evolution():
+    h1 = [(((0, "X"), (2, "Z")), 0.5), (((1, "Y"),), -0.25)]
+    ops = [["alloc", 5]] + [g("H", [[q]]) for q in range(5)]
+    ops += [g("TimeEvolution", [[0, 1, 2, 3]], te(0.3, h1))]
+    ops += [g("TimeEvolution", [[0, 1, 2, 3]], te(0.3, h1))]  # equal only by identity: no merge
+    ops += [g("TimeEvolution", [[1, 2]], te(0.7, [((), 1.5)]))]  # identity only: a phase gate
+    ops += [g("TimeEvolution", [[2, 3]], te(0.2, [(((0, "Z"), (1, "Z")), 1.0), ((), 0.5)]), ctrls=[0])]
+    ops += [g("TimeEvolution", [[4, 0, 3]], te(-0.6, [(((2, "X"), (0, "Y")), 0.75), (((1, "Z"),), 2.0)]))]
+    ops += [["dagger"], g("TimeEvolution", [[3]], te(0.4, [(((0, "X"),), 1.0)])), g("H", [[3]]), ["dagger_end"]]
+    ops += [["compute"], g("TimeEvolution", [[1, 4]], te(0.1, [(((0, "Y"), (1, "Y")), 0.3)])), ["compute_end"]]
+    ops += [g("Z", [[2]], ctrls=[1]), ["uncompute"]]
+    ops += [["measure", q] for q in range(5)] + [["dealloc", q] for q in range(5)]
+    return ops
+
+
 # --- seeded random programs -----------------------------------------------------------------
 ANGLE_POOL = [0.0, PI / 4, PI / 2, PI, 2 * PI, 3 * PI, 4 * PI - 1e-13, 4 * PI, -PI / 2, 1e-13]
 
@@ -253,6 +273,7 @@ PROGRAMS = {
     "realloc": prog_realloc(),
     "layers6": prog_layers(),
     "identity_multiqubit": prog_identity_multiqubit(),
+    "time_evolution": prog_time_evolution(),
 }
 for _s in range(12):
     PROGRAMS[f"random{_s}"] = prog_random(100 + _s, n=4 + _s % 3)
@@ -267,8 +288,8 @@ def run_reference(ops):
     from projectq.libs import math as qmath
     from projectq.meta import Compute, Control, Dagger, Uncompute
     from projectq.ops import (
-        QFT, Barrier, H, MatrixGate, Measure, Ph, R, Rx, Rxx, Ry, Ryy, Rz, Rzz, S, Sdag, SqrtSwap, SqrtX, Swap, T,
-        Tdag, X, Y, Z, get_inverse,
+        QFT, Barrier, H, MatrixGate, Measure, Ph, QubitOperator, R, Rx, Rxx, Ry, Ryy, Rz, Rzz, S, Sdag, SqrtSwap,
+        SqrtX, Swap, T, Tdag, TimeEvolution, X, Y, Z, get_inverse,
     )
     from projectq.types import Qureg
 
@@ -294,6 +315,11 @@ def run_reference(ops):
             return fixed[name]
         if name == "Matrix":
             return MatrixGate([[complex(re, im) for re, im in row] for row in params])
+        if name == "TimeEvolution":
+            ham = QubitOperator()
+            for term, c in params[1]:
+                ham += QubitOperator(tuple(tuple(lp) for lp in term), c)
+            return TimeEvolution(params[0], ham)
         return param[name](*params)
 
     def ctl(ctrls, state):

@@ -102,7 +102,7 @@ def test_qft_expansion_order():
     assert gates == [([2], []), ([2], [1]), ([2], [0]), ([1], []), ([1], [0]), ([0], [])]
 
 
-@pytest.mark.parametrize("name", ["layers6", "qft", "math", "compute_uncompute", "random3"])
+@pytest.mark.parametrize("name", ["layers6", "qft", "math", "compute_uncompute", "time_evolution", "random3"])
 def test_default_list_runs_on_the_oracle_like_the_recorded_calls(name):
     """The compiled program and the reference's recorded calls, each run on the oracle with the
     same seed, measure the same outcomes and end in the same state (one amplitude once every
@@ -128,6 +128,9 @@ def play_call(sim, call):
     if kind == "apply_controlled_gate":
         m = np.array([[complex(re, im) for re, im in row] for row in call[1]])
         return sim.apply_controlled_gate(m, call[2], call[3])
+    if kind == "emulate_time_evolution":
+        terms = [([tuple(lp) for lp in term], c) for term, c in call[1]]
+        return sim.emulate_time_evolution(terms, call[2], call[3], call[4])
     if kind == "emulate_math":
         table, regs = call[1], call[2]
         lookup = {}
@@ -170,7 +173,7 @@ def test_default_list_on_the_gpu(precision):
     from oracle.cpu_sim import OracleSimulator
     from projectq_amd import Simulator
 
-    for name in ("layers6", "qft", "math", "random5"):
+    for name in ("layers6", "qft", "math", "time_evolution", "random5"):
         ops = [op for op in TRACES[name]["ops"] if op[0] not in ("dealloc", "measure")]
         calls = [c for c in TRACES[name]["calls"] if c[0] not in ("deallocate_qubit", "measure_qubits")]
         h = Simulator(3, precision=precision)
