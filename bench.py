@@ -14,7 +14,9 @@ Also reported:
   in-repo oracle where that build is absent, on a bounded sample of the same circuit;
 * batched_variational: 512 variational ansatze with 100-term Pauli energies in one hipGraph launch;
 * qft_rotations_adder_30q: the 30-qubit QFT + rotations + AddQuantum + probabilities + measurement
-  workload in complex128, with bit-exact and normalisation checks.
+  workload in complex128, with bit-exact and normalisation checks;
+* default_engine_list_20q: the 20-qubit random-layer circuit written as a ProjectQ program and
+  compiled by the default engine list in front of the GPU simulator (host compiler time included).
 """
 
 import argparse
@@ -278,6 +280,47 @@ def qft_rotations_adder(args, n=30):
     }
 
 
+def default_engine_list(args, n=20, layers=20):
+    """The 20-qubit, 20-layer random-layer circuit (10 CNOTs per layer, measure all) written as a
+    ProjectQ program and compiled by the default engine list (projectq_amd.compiler) in front of
+    the GPU simulator: once as the reference runs it by default (complex128, run() after every
+    gate) and once with gate_fusion=True into complex64 window fusion with the learned planner.
+    Times include the host-side compiler."""
+    from projectq_amd import Simulator, circuits
+    from projectq_amd import compiler as C
+
+    ops = circuits.random_layers_program(n, layers, 10, args.seed)
+    n_gates = sum(1 for op in ops if op[0] == "gate")
+    out = {"workload": f"{n} qubits, {layers} layers (H, random Rx/Ry/Rz, 10 CNOTs), measure all; "
+           "default engine list (TagRemover, LocalOptimizer(10), AutoReplacer, TagRemover, LocalOptimizer(10))",
+           "gates": n_gates}
+    for label, kw, fusion in (("c128_run_each_gate", dict(precision="c128"), False),
+                              ("c64_window_learned", dict(precision="c64", fusion="window", planner="learned"), True)):
+        sim = Simulator(args.seed, capacity_qubits=n, **kw)
+        calls = {"apply_controlled_gate": 0}
+
+        class Counting:
+            def __getattr__(self, name):
+                fn = getattr(sim, name)
+                if name == "apply_controlled_gate":
+                    def counted(*a):
+                        calls["apply_controlled_gate"] += 1
+                        return fn(*a)
+                    return counted
+                return fn
+
+        t0 = time.perf_counter()
+        eng = C.run_program(ops, C.SimulatorEngine(Counting(), gate_fusion=fusion))
+        sim.synchronize()
+        dt = time.perf_counter() - t0
+        out[label] = {"s": round(dt, 4), "gates_per_s": round(n_gates / dt, 1),
+                      "backend_gate_calls": calls["apply_controlled_gate"],
+                      "outcomes": "".join("1" if eng.results[q] else "0" for q in range(n))}
+        del eng, sim
+    log(f"default engine list: {out}")
+    return out
+
+
 def kernel_name(args, plan):
     """The gate-pass kernel the plan ran: the tile launcher uses the software-pipelined variant for
     complex64 tiles of 2^12 amplitudes with 4 register bits."""
@@ -337,6 +380,7 @@ def main():
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--calibrate-copy", action="store_true", help="add a known-size D2D copy (counter scaling)")
     ap.add_argument("--no-qft-adder", action="store_true", help="skip the 30-qubit QFT + rotations + AddQuantum extra")
+    ap.add_argument("--no-engine-list", action="store_true", help="skip the default-engine-list extra")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -423,6 +467,9 @@ def main():
 
     if not args.no_qft_adder:
         out["qft_rotations_adder_30q"] = qft_rotations_adder(args)
+
+    if not args.no_engine_list:
+        out["default_engine_list_20q"] = default_engine_list(args)
 
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(args)

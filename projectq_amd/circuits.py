@@ -44,6 +44,25 @@ def random_layers(n, layers, cnots_per_layer, seed, measure=True):
     return cmds
 
 
+def random_layers_program(n, layers, cnots_per_layer, seed, measure=True):
+    """`random_layers` as a program of `projectq_amd.compiler.run_program` (same seed, same
+    gates): what a ProjectQ user writes, before the default engine list optimises it."""
+    rng = np.random.default_rng(seed)
+    ops = [["alloc", n]]
+    names = ("Rx", "Ry", "Rz")
+    for _ in range(layers):
+        ops += [["gate", "H", [], [[q]], [], None] for q in range(n)]
+        kinds = rng.integers(0, 3, size=n)
+        angles = rng.uniform(0.0, 2.0 * math.pi, size=n)
+        ops += [["gate", names[kinds[q]], [float(angles[q])], [[q]], [], None] for q in range(n)]
+        for _ in range(cnots_per_layer):
+            c, t = rng.choice(n, size=2, replace=False)
+            ops.append(["gate", "X", [], [[int(t)]], [int(c)], None])
+    if measure:
+        ops += [["measure", q] for q in range(n)]
+    return ops
+
+
 def qft(qubits):
     """QFT as H and controlled-R in the order of projectq/setups/decompositions/
     qft2crandhadamard.py:35-39."""

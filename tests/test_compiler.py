@@ -186,3 +186,17 @@ def test_default_list_on_the_gpu(precision):
         got, want = by_qubit_id(*h.cheat_numpy()), by_qubit_id(*o.cheat_numpy())
         tol = 1e-12 if precision == "c128" else 2e-6
         assert np.max(np.abs(got - want)) <= tol, name
+
+
+def test_random_layers_program_is_the_command_stream():
+    """The program form of the random-layer circuit holds the same gates as the command form."""
+    from projectq_amd import circuits
+
+    ops = circuits.random_layers_program(7, 3, 5, 9)
+    cmds = circuits.random_layers(7, 3, 5, 9)
+    progs = [o for o in ops if o[0] == "gate"]
+    gates = [c for c in cmds if c[0] == "gate"]
+    assert len(progs) == len(gates) == 3 * (2 * 7 + 5)
+    for o, c in zip(progs, gates):
+        assert np.array_equal(C.make_gate(o[1], o[2]).matrix, c[1])
+        assert (o[3], o[4]) == ([list(c[2])], list(c[3]))
