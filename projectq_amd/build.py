@@ -30,6 +30,10 @@ FLAGS = [
     "-target-feature",
     "-Xclang",
     "-packed-fp32-ops",
+    # uniform branches (the register-op dispatch) keep their CFG: no flow blocks whose phis copy
+    # the amplitude registers
+    "-mllvm",
+    "-structurizecfg-skip-uniform-regions=true",
     "-I",
     os.path.join(ROOT, "include"),
     "-I",

@@ -236,12 +236,26 @@ __device__ __forceinline__ void tile_apply(amp_t<R> *__restrict__ tile, uint32_t
 // bit i.  C is the op's control among the register bits (-1: none); it is a compile-time
 // constant, so a controlled op touches only the indices with bit C set and a controlled X is a
 // renaming of registers.
+//
+// Controls on tile bits outside the register bits hold for some threads and not others.  They do
+// not branch: ops with such controls run the PRED form, in which every lane runs the op and lanes
+// whose controls fail (`act` false) apply the identity -- the matrix is blended with the
+// identity, or the results are selected.  With no divergent branch in the op loop the
+// op dispatch stays uniform control flow, which the compiler lays out without copying the
+// amplitude registers at every merge (with -structurizecfg-skip-uniform-regions).
 __host__ __device__ constexpr bool reg_ctrl_ok(int i, int C) { return C < 0 || ((i >> (C < 0 ? 0 : C)) & 1); }
 
-template <typename R, int RB, int C, int B>
-__device__ __forceinline__ void rop_general(R *vr, R *vi, const amp_t<R> *__restrict__ m) {
-  const R ar = m[0].x, ai = m[0].y, br = m[1].x, bi = m[1].y;
-  const R cr = m[2].x, ci = m[2].y, dr = m[3].x, di = m[3].y;
+template <typename R>
+__device__ __forceinline__ R blend(bool act, R v, R id) { return act ? v : id; }
+
+template <typename R, int RB, bool PRED, int C, int B>
+__device__ __forceinline__ void rop_general(R *vr, R *vi, const amp_t<R> *__restrict__ m, bool act) {
+  R ar = m[0].x, ai = m[0].y, br = m[1].x, bi = m[1].y;
+  R cr = m[2].x, ci = m[2].y, dr = m[3].x, di = m[3].y;
+  if constexpr (PRED) {
+    ar = blend<R>(act, ar, 1), ai = blend<R>(act, ai, 0), br = blend<R>(act, br, 0), bi = blend<R>(act, bi, 0);
+    cr = blend<R>(act, cr, 0), ci = blend<R>(act, ci, 0), dr = blend<R>(act, dr, 1), di = blend<R>(act, di, 0);
+  }
 #pragma unroll
   for (int i = 0; i < (1 << RB); ++i) {
     if ((i & (1 << B)) || !reg_ctrl_ok(i, C)) continue;
@@ -253,9 +267,10 @@ __device__ __forceinline__ void rop_general(R *vr, R *vi, const amp_t<R> *__rest
     vi[j] = fma(cr, xi, fma(ci, xr, fma(dr, yi, di * yr)));
   }
 }
-template <typename R, int RB, int C, int B>
-__device__ __forceinline__ void rop_real(R *vr, R *vi, const amp_t<R> *__restrict__ m) {
-  const R a = m[0].x, b = m[1].x, c = m[2].x, d = m[3].x;
+template <typename R, int RB, bool PRED, int C, int B>
+__device__ __forceinline__ void rop_real(R *vr, R *vi, const amp_t<R> *__restrict__ m, bool act) {
+  R a = m[0].x, b = m[1].x, c = m[2].x, d = m[3].x;
+  if constexpr (PRED) a = blend<R>(act, a, 1), b = blend<R>(act, b, 0), c = blend<R>(act, c, 0), d = blend<R>(act, d, 1);
 #pragma unroll
   for (int i = 0; i < (1 << RB); ++i) {
     if ((i & (1 << B)) || !reg_ctrl_ok(i, C)) continue;
@@ -267,9 +282,10 @@ __device__ __forceinline__ void rop_real(R *vr, R *vi, const amp_t<R> *__restric
     vi[j] = fma(c, xi, d * yi);
   }
 }
-template <typename R, int RB, int C, int B>
-__device__ __forceinline__ void rop_diag(R *vr, R *vi, const amp_t<R> *__restrict__ m) {
-  const R ar = m[0].x, ai = m[0].y, dr = m[3].x, di = m[3].y;
+template <typename R, int RB, bool PRED, int C, int B>
+__device__ __forceinline__ void rop_diag(R *vr, R *vi, const amp_t<R> *__restrict__ m, bool act) {
+  R ar = m[0].x, ai = m[0].y, dr = m[3].x, di = m[3].y;
+  if constexpr (PRED) ar = blend<R>(act, ar, 1), ai = blend<R>(act, ai, 0), dr = blend<R>(act, dr, 1), di = blend<R>(act, di, 0);
 #pragma unroll
   for (int i = 0; i < (1 << RB); ++i) {
     if (!reg_ctrl_ok(i, C)) continue;
@@ -279,22 +295,37 @@ __device__ __forceinline__ void rop_diag(R *vr, R *vi, const amp_t<R> *__restric
     vi[i] = fma(fr, xi, fi * xr);
   }
 }
-template <typename R, int RB, int C, int B>
-__device__ __forceinline__ void rop_anti(R *vr, R *vi, const amp_t<R> *__restrict__ m) {
+template <typename R, int RB, bool PRED, int C, int B>
+__device__ __forceinline__ void rop_anti(R *vr, R *vi, const amp_t<R> *__restrict__ m, bool act) {
   const R br = m[1].x, bi = m[1].y, cr = m[2].x, ci = m[2].y;
 #pragma unroll
   for (int i = 0; i < (1 << RB); ++i) {
     if ((i & (1 << B)) || !reg_ctrl_ok(i, C)) continue;
     const int j = i | (1 << B);
     const R xr = vr[i], xi = vi[i], yr = vr[j], yi = vi[j];
-    vr[i] = fma(br, yr, -bi * yi);
-    vi[i] = fma(br, yi, bi * yr);
-    vr[j] = fma(cr, xr, -ci * xi);
-    vi[j] = fma(cr, xi, ci * xr);
+    const R nr_i = fma(br, yr, -bi * yi), ni_i = fma(br, yi, bi * yr);
+    const R nr_j = fma(cr, xr, -ci * xi), ni_j = fma(cr, xi, ci * xr);
+    vr[i] = PRED ? blend<R>(act, nr_i, xr) : nr_i;
+    vi[i] = PRED ? blend<R>(act, ni_i, xi) : ni_i;
+    vr[j] = PRED ? blend<R>(act, nr_j, yr) : nr_j;
+    vi[j] = PRED ? blend<R>(act, ni_j, yi) : ni_j;
   }
 }
-template <typename R, int RB, int C, int B>
-__device__ __forceinline__ void rop_x(R *vr, R *vi, const amp_t<R> *__restrict__) {
+template <typename R, int RB, bool PRED, int C, int B>
+__device__ __forceinline__ void rop_x(R *vr, R *vi, const amp_t<R> *__restrict__, bool act) {
+  if constexpr (PRED) {
+#pragma unroll
+    for (int i = 0; i < (1 << RB); ++i) {
+      if ((i & (1 << B)) || !reg_ctrl_ok(i, C)) continue;
+      const int j = i | (1 << B);
+      const R xr = vr[i], xi = vi[i], yr = vr[j], yi = vi[j];
+      vr[i] = blend<R>(act, yr, xr);
+      vi[i] = blend<R>(act, yi, xi);
+      vr[j] = blend<R>(act, xr, yr);
+      vi[j] = blend<R>(act, xi, yi);
+    }
+    return;
+  }
 #pragma unroll
   for (int i = 0; i < (1 << RB); ++i) {
     if ((i & (1 << B)) || !reg_ctrl_ok(i, C)) continue;
@@ -306,8 +337,15 @@ __device__ __forceinline__ void rop_x(R *vr, R *vi, const amp_t<R> *__restrict__
     vi[j] = xi;
   }
 }
-template <typename R, int RB, int C, int B0, int B1>
-__device__ __forceinline__ void rop_general2(R *vr, R *vi, const amp_t<R> *__restrict__ m) {
+template <typename R, int RB, bool PRED, int C, int B0, int B1>
+__device__ __forceinline__ void rop_general2(R *vr, R *vi, const amp_t<R> *__restrict__ m, bool act) {
+  R mr[16], mi[16];
+#pragma unroll
+  for (int k = 0; k < 16; ++k) {
+    mr[k] = m[k].x;
+    mi[k] = m[k].y;
+    if constexpr (PRED) mr[k] = blend<R>(act, mr[k], (k % 5) == 0 ? R(1) : R(0)), mi[k] = blend<R>(act, mi[k], 0);
+  }
 #pragma unroll
   for (int i = 0; i < (1 << RB); ++i) {
     if ((i & ((1 << B0) | (1 << B1))) || !reg_ctrl_ok(i, C)) continue;
@@ -323,25 +361,30 @@ __device__ __forceinline__ void rop_general2(R *vr, R *vi, const amp_t<R> *__res
       R sr = 0, si = 0;
 #pragma unroll
       for (int c = 0; c < 4; ++c) {
-        const amp_t<R> mc = m[r * 4 + c];
-        sr = fma(mc.x, xr[c], fma(-mc.y, xi[c], sr));
-        si = fma(mc.x, xi[c], fma(mc.y, xr[c], si));
+        sr = fma(mr[r * 4 + c], xr[c], fma(-mi[r * 4 + c], xi[c], sr));
+        si = fma(mr[r * 4 + c], xi[c], fma(mi[r * 4 + c], xr[c], si));
       }
       vr[idx[r]] = sr;
       vi[idx[r]] = si;
     }
   }
 }
-template <typename R, int RB, int C, int B0, int B1>
-__device__ __forceinline__ void rop_diag2(R *vr, R *vi, const amp_t<R> *__restrict__ m) {
+template <typename R, int RB, bool PRED, int C, int B0, int B1>
+__device__ __forceinline__ void rop_diag2(R *vr, R *vi, const amp_t<R> *__restrict__ m, bool act) {
+  R fr[4], fi[4];
+#pragma unroll
+  for (int s = 0; s < 4; ++s) {
+    fr[s] = m[s * 5].x;  // diagonal element s of the 4x4 matrix
+    fi[s] = m[s * 5].y;
+    if constexpr (PRED) fr[s] = blend<R>(act, fr[s], 1), fi[s] = blend<R>(act, fi[s], 0);
+  }
 #pragma unroll
   for (int i = 0; i < (1 << RB); ++i) {
     if (!reg_ctrl_ok(i, C)) continue;
     const int s = ((i >> B0) & 1) | (((i >> B1) & 1) << 1);
-    const amp_t<R> f = m[s * 5];  // diagonal element s of the 4x4 matrix
     const R xr = vr[i], xi = vi[i];
-    vr[i] = fma(f.x, xr, -f.y * xi);
-    vi[i] = fma(f.x, xi, f.y * xr);
+    vr[i] = fma(fr[s], xr, -fi[s] * xi);
+    vi[i] = fma(fr[s], xi, fi[s] * xr);
   }
 }
 
@@ -349,7 +392,7 @@ __device__ __forceinline__ void rop_diag2(R *vr, R *vi, const amp_t<R> *__restri
 // none), in one function so that the amplitude registers stay in registers.  Combinations the
 // planner never emits (a control on a target, bits >= RB) do nothing.
 #define QS_R1(FN, C, B)                                          \
-  if constexpr ((B) < RB && (C) < RB && (C) != (B)) FN<R, RB, C, B>(vr, vi, m);
+  if constexpr ((B) < RB && (C) < RB && (C) != (B)) FN<R, RB, PRED, C, B>(vr, vi, m, act);
 #define QS_R1C(FN, B)                                            \
   switch (cb) {                                                  \
     case 0: QS_R1(FN, -1, B) break;                              \
@@ -368,7 +411,7 @@ __device__ __forceinline__ void rop_diag2(R *vr, R *vi, const amp_t<R> *__restri
     default: QS_R1C(FN, 4) break;                                \
   }
 #define QS_R2(FN, C, B0, B1)                                                            \
-  if constexpr ((B1) < RB && (C) < RB && (C) != (B0) && (C) != (B1)) FN<R, RB, C, B0, B1>(vr, vi, m);
+  if constexpr ((B1) < RB && (C) < RB && (C) != (B0) && (C) != (B1)) FN<R, RB, PRED, C, B0, B1>(vr, vi, m, act);
 #define QS_R2C(FN, B0, B1)                                       \
   switch (cb) {                                                  \
     case 0: QS_R2(FN, -1, B0, B1) break;                         \
@@ -392,8 +435,8 @@ __device__ __forceinline__ void rop_diag2(R *vr, R *vi, const amp_t<R> *__restri
     default: QS_R2C(FN, 3, 4) break;                             \
   }
 
-template <typename R, int RB>
-__device__ __forceinline__ void reg_op(uint32_t code, R *vr, R *vi, const amp_t<R> *__restrict__ m) {
+template <typename R, int RB, bool PRED>
+__device__ __forceinline__ void reg_op(uint32_t code, R *vr, R *vi, const amp_t<R> *__restrict__ m, bool act) {
   const uint32_t kind = (code >> 8) & 15u, b0 = code & 15u, b1 = (code >> 4) & 15u, cb = code >> 12;
   switch (kind) {
     case kOp1General: QS_R1B(rop_general) break;
@@ -442,17 +485,25 @@ __device__ __forceinline__ void reg_stage(amp_t<R> *__restrict__ tile, uint32_t 
     // Two ops per turn of the loop: the register allocator copies the amplitudes between two
     // register sets at the loop's back edge, once per turn rather than once per op.
     for (uint32_t o = ob; o < oe; o += 2) {
+      // global controls are uniform over the block (a branch); tile-bit controls are per lane
+      // (a predicate inside the op)
       {
         const RegOp *op = rops + o;
         const uint64_t gc = op->gcmask;
         const uint32_t lc = op->lcmask;
-        if ((base & gc) == gc && (lb & lc) == lc) reg_op<R, RB>(op->code, vr, vi, mats + op->mat);
+        if ((base & gc) == gc) {
+          if (lc == 0) reg_op<R, RB, false>(op->code, vr, vi, mats + op->mat, true);
+          else reg_op<R, RB, true>(op->code, vr, vi, mats + op->mat, (lb & lc) == lc);
+        }
       }
       if (o + 1 < oe) {
         const RegOp *op = rops + o + 1;
         const uint64_t gc = op->gcmask;
         const uint32_t lc = op->lcmask;
-        if ((base & gc) == gc && (lb & lc) == lc) reg_op<R, RB>(op->code, vr, vi, mats + op->mat);
+        if ((base & gc) == gc) {
+          if (lc == 0) reg_op<R, RB, false>(op->code, vr, vi, mats + op->mat, true);
+          else reg_op<R, RB, true>(op->code, vr, vi, mats + op->mat, (lb & lc) == lc);
+        }
       }
     }
     // Recompute the LDS addresses for the write-back instead of keeping 2^RB of them live across
