@@ -349,7 +349,7 @@ def calibrate_copy(nbytes=1 << 31):
     log(f"calibration copy: {nbytes} bytes read and written")
 
 
-TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "k_tile_pass_random32_c64_traffic.json")
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "k_tile_pass_pf_random32_c64_traffic.json")
 
 
 def measured_traffic(args, n):
