@@ -278,3 +278,27 @@ int emu_run(unsigned n, unsigned t, unsigned lmin, unsigned rb, int c64, int n_g
   }
 }
 }
+
+// Register-stage ops of the plan, counted by kind (0..6): counts[kind] for ops without tile-bit
+// controls, counts[7 + kind] for ops with them (lcmask != 0, the kernel's predicated forms).
+extern "C" int emu_op_kinds(unsigned n, unsigned t, unsigned lmin, unsigned rb, int c64, int n_gates, const uint32_t *k,
+                            const uint32_t *tpos, const uint64_t *cmask, const double *mats, int *counts, char *err,
+                            int err_len) {
+  try {
+    auto gates = qs::merge_single_qubit_runs(read_gates(n_gates, k, tpos, cmask, mats));
+    auto passes = qs::schedule_tiles(gates, qs::TileShape{n, t, lmin});
+    for (int i = 0; i < 14; ++i) counts[i] = 0;
+    for (auto &ps : passes) {
+      qs::TileProgram prog;
+      qs::build_tile_pass(gates, ps, std::min(t, n), rb, c64 != 0, prog);
+      for (const auto &op : prog.rops) {
+        const unsigned kind = (op.code >> 8) & 15u;
+        if (kind < 7) counts[(op.lcmask != 0 ? 7 : 0) + kind] += 1;
+      }
+    }
+    return 0;
+  } catch (const std::exception &e) {
+    std::strncpy(err, e.what(), err_len - 1);
+    return 2;
+  }
+}

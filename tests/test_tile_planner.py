@@ -213,3 +213,80 @@ def test_register_stages_spread_over_lds_banks(emu):
     degrees = {s: bank_conflict_degree(slot, s, 12, 32, 32) for s in set(stages)}
     mean = sum(degrees[s] for s in stages) / len(stages)
     assert mean <= 1.15, mean
+
+
+def lane_controlled_mix(n, n_gates, seed):
+    """Controlled gates of every register-op kind (general, real, diagonal, anti-diagonal, X,
+    2-qubit general, 2-qubit diagonal) with 1-2 controls on random qubits, so that many controls
+    fall on tile bits outside a stage's register bits."""
+    rng = np.random.default_rng(seed)
+    makers = [
+        lambda a: (G.Rx(a), 1), lambda a: (G.H, 1), lambda a: (G.Ry(a), 1), lambda a: (G.Rz(a), 1), lambda a: (G.R(a), 1),
+        lambda a: (G.Y, 1), lambda a: (G.X, 1), lambda a: (G.Rxx(a), 2), lambda a: (G.Rzz(a), 2),
+    ]
+    cmds = circuits.allocate(n) + [commands.gate(G.H, [q]) for q in range(n)]
+    for _ in range(n_gates):
+        m, k = makers[int(rng.integers(len(makers)))](float(rng.uniform(0, 4 * np.pi)))
+        picks = [int(q) for q in rng.choice(n, size=k + int(rng.integers(1, 3)), replace=False)]
+        cmds.append(commands.gate(m, picks[:k], picks[k:]))
+    return cmds
+
+
+def op_kind_counts(lib, n, t, lmin, rb, cmds, c64):
+    ks, tpos, cm, mats = [], [], [], []
+    for c in cmds:
+        if c[0] != "gate":
+            continue
+        ks.append(len(c[2]))
+        tpos.extend(c[2])
+        cm.append(sum(1 << q for q in c[3]))
+        mats.append(np.asarray(c[1], dtype=np.complex128).reshape(-1))
+    k = np.asarray(ks, dtype=np.uint32)
+    tp = np.asarray(tpos, dtype=np.uint32)
+    cmask = np.asarray(cm, dtype=np.uint64)
+    mflat = np.ascontiguousarray(np.concatenate(mats))
+    counts = np.zeros(14, dtype=np.int32)
+    err = ctypes.create_string_buffer(256)
+    rc = lib.emu_op_kinds(
+        ctypes.c_uint(n), ctypes.c_uint(t), ctypes.c_uint(lmin), ctypes.c_uint(rb), ctypes.c_int(int(c64)),
+        ctypes.c_int(len(ks)), k.ctypes.data_as(ctypes.c_void_p), tp.ctypes.data_as(ctypes.c_void_p),
+        cmask.ctypes.data_as(ctypes.c_void_p), mflat.ctypes.data_as(ctypes.c_void_p),
+        counts.ctypes.data_as(ctypes.c_void_p), err, ctypes.c_int(256),
+    )
+    assert rc == 0, err.value.decode()
+    return counts[:7], counts[7:]
+
+
+LANE_MIX = dict(n=15, n_gates=400, seed=21)
+
+
+@pytest.mark.parametrize("c64", [True, False])
+def test_lane_controlled_mix_covers_every_predicated_op_kind(emu, c64):
+    """The circuit of the GPU test below plans (t=12, RB=4, L=4) into register ops of all seven
+    kinds both with and without tile-bit controls, so that test runs every plain and every
+    predicated op form of the kernel; the CPU replay of the plan matches the unfused oracle."""
+    cmds = lane_controlled_mix(**LANE_MIX)
+    plain, lane = op_kind_counts(emu, LANE_MIX["n"], 12, 4, 4, cmds, c64)
+    assert (lane > 0).all(), lane
+    assert (plain > 0).all(), plain
+    want = oracle_state(cmds, LANE_MIX["n"])
+    got, _ = run_emu(emu, LANE_MIX["n"], 12, 4, cmds, start_state(LANE_MIX["n"]), 4, c64)
+    assert np.max(np.abs(got - want)) <= 1e-12
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("precision", ["c128", "c64"])
+def test_lane_controlled_mix_gpu(precision):
+    """Every register-op kind with and without tile-bit (lane) controls on the GPU tile pass."""
+    from projectq_amd import Simulator
+
+    n = LANE_MIX["n"]
+    cmds = lane_controlled_mix(**LANE_MIX)
+    want = oracle_state(cmds, n)
+    sim = Simulator(5, precision=precision, fusion="window", planner="tile")
+    sim.set_tile_params(12, 4, 4)
+    commands.execute(sim, cmds, gate_fusion=True)
+    sim.run()
+    got = sim.cheat_numpy()[1]
+    tol = 1e-12 if precision == "c128" else 2e-5
+    assert np.max(np.abs(got - want)) <= tol
