@@ -28,7 +28,9 @@ Restated pieces and the reference code each follows:
   (setups/decompositions/controlstate.py) and barriers (setups/decompositions/barrier.py);
 * the meta sections Control, Compute/Uncompute (without qubit allocation inside) and Dagger
   (projectq/meta/_control.py, _compute.py, _dagger.py);
-* `TimeEvolution` with the index renumbering of its `|` operator (projectq/ops/_time_evolution.py);
+* `TimeEvolution` and single-term `QubitOperator` gates with the rewriting of their `|` operators
+  (projectq/ops/_time_evolution.py, _qubit_operator.py:224-343), the Entangle and QubitOperator
+  rules (setups/decompositions/entangle.py, qubitop2onequbit.py);
 * `SimulatorEngine` -- the backend's `is_available`, `_handle` and `receive`.
 
 The optimizer's list operations are kept step for step, including two behaviours of the reference
@@ -38,6 +40,7 @@ encountered" rather than raising.  Programs whose reference run raises reproduce
 exception type.
 """
 
+import cmath
 import math
 from contextlib import contextmanager
 
@@ -421,6 +424,88 @@ class ComparatorQuantumGate(BasicMathGate):
         return G.comparator_quantum()
 
 
+class MultiplyQuantumGate(BasicMathGate):
+    name = "MultiplyQuantum"
+
+    def get_inverse(self):
+        return InverseMultiplyQuantumGate()
+
+    def math_function(self, quregs):
+        return G.multiply_quantum()
+
+
+class InverseMultiplyQuantumGate(BasicMathGate):
+    name = "_InverseMultiplyQuantum"
+
+    def math_function(self, quregs):  # _gates.py:541-551
+        return G._tag(lambda v: [v[0], v[1], v[2] - v[0] * v[1]], _mathfuncs.INVERSE_MULTIPLY_QUANTUM)
+
+
+class DivideQuantumGate(BasicMathGate):
+    name = "DivideQuantum"
+
+    def get_inverse(self):
+        return InverseDivideQuantumGate()
+
+    def math_function(self, quregs):
+        return G.divide_quantum()
+
+
+class InverseDivideQuantumGate(BasicMathGate):
+    name = "_InverseDivideQuantum"
+
+    def math_function(self, quregs):  # _gates.py:467-486
+        def inverse_division(v):
+            remainder, quotient, divisor = v
+            if divisor == 0:
+                return [quotient, remainder, divisor]
+            return [remainder + quotient * divisor, 0, divisor]
+
+        return G._tag(inverse_division, _mathfuncs.INVERSE_DIVIDE_QUANTUM)
+
+
+class EntangleGate(BasicGate):
+    """H on the first qubit, then X on every other one controlled by it (_gates.py:245-255)."""
+
+    name = "Entangle"
+
+
+class QubitOperatorGate(BasicGate):
+    """A single Pauli string with a unit coefficient used as a gate (projectq/ops/_qubit_operator.py
+    __or__, get_inverse).  Like the reference's QubitOperator it takes BasicGate's equality: any
+    two compare equal, so the optimizer cancels two adjacent ones on the same qubits whatever
+    their Pauli strings."""
+
+    name = "QubitOperator"
+
+    def __init__(self, term, coefficient):
+        self.term = tuple(sorted(((int(q), str(p)) for q, p in term), key=lambda lp: lp[0]))
+        self.coefficient = complex(coefficient)
+
+    def get_inverse(self):
+        if not abs(self.coefficient) < 1 - 1e-12 and not abs(self.coefficient) > 1 + 1e-12:
+            return QubitOperatorGate(self.term, self.coefficient ** (-1))
+        raise NotInvertible("BasicGate: No get_inverse() implemented.")
+
+    def commands_for(self, qureg):
+        """What `QubitOperator(...) | qureg` emits: Ph for the identity, the Pauli and Ph for one
+        qubit, else the gate on the qubits it acts on, renumbered."""
+        c = self.coefficient
+        if abs(c) < 1 - 1e-12 or abs(c) > 1 + 1e-12:
+            raise TypeError("abs(coefficient) != 1.")
+        phase = cmath.phase(c)
+        if self.term == ():
+            return [(Ph(phase), [qureg[0]])]
+        used = sorted({q for q, _ in self.term})
+        if used[-1] >= len(qureg):
+            raise ValueError("QubitOperator acts on more qubits than the gate is applied to.")
+        if len(self.term) == 1:
+            q, p = self.term[0]
+            return [({"X": XGate, "Y": YGate, "Z": ZGate}[p](), [qureg[q]]), (Ph(phase), [qureg[q]])]
+        pos = {q: i for i, q in enumerate(used)}
+        return [(QubitOperatorGate(tuple((pos[q], p) for q, p in self.term), c), [qureg[q] for q in used])]
+
+
 class NotHermitianOperatorError(ValueError):
     pass
 
@@ -472,13 +557,13 @@ class TimeEvolution(BasicGate):
         the first qubit for an identity-only H, else the gate on the qubits H acts on, with
         the term indices renumbered to positions in that list."""
         if len(self.terms) == 1 and () in self.terms:
-            return Ph(-1 * self.time * self.terms[()]), [qureg[0]]
+            return [(Ph(-1 * self.time * self.terms[()]), [qureg[0]])]
         used = sorted({q for term in self.terms for q, _ in term})
         if used[-1] >= len(qureg):
             raise ValueError("hamiltonian acts on more qubits than the gate is applied to.")
         pos = {q: i for i, q in enumerate(used)}
         terms = [(tuple((pos[q], p) for q, p in term), c) for term, c in self.terms.items()]
-        return TimeEvolution(self.time, terms), [qureg[q] for q in used]
+        return [(TimeEvolution(self.time, terms), [qureg[q] for q in used])]
 
 
 # shortcuts, as projectq.ops / projectq.libs.math export them
@@ -763,6 +848,10 @@ class AutoReplacer(Engine):
             rule = _inverse_rule(_decompose_qft)
         elif isinstance(cmd.gate, BarrierGate):
             rule = _decompose_barrier
+        elif isinstance(cmd.gate, EntangleGate):
+            rule = _decompose_entangle
+        elif isinstance(cmd.gate, QubitOperatorGate) and len(cmd.control_qubits) <= 1:
+            rule = _decompose_qubitop
         else:
             raise NoGateDecompositionError(f"No replacement found for {cmd!r}!")
         old_tags = list(cmd.tags)
@@ -810,6 +899,26 @@ def _decompose_controlstate(cmd, fe):
 
 def _decompose_barrier(cmd, fe):
     """barrier.py: a barrier the backend does not take is dropped."""
+
+
+def _decompose_entangle(cmd, fe):
+    """entangle.py: H on the first qubit, then X on each other qubit controlled by it."""
+    qureg = cmd.qubits[0]
+    with fe.control(cmd.control_qubits):
+        fe.apply(HGate(), [qureg[0]])
+        with fe.control([qureg[0]]):
+            for q in qureg[1:]:
+                fe.apply(XGate(), [q])
+
+
+def _decompose_qubitop(cmd, fe):
+    """qubitop2onequbit.py: Ph(phase of the coefficient) on the first qubit, then the Paulis."""
+    qureg = cmd.qubits[0]
+    gate = cmd.gate
+    with fe.control(cmd.control_qubits):
+        fe.apply(Ph(cmath.phase(gate.coefficient)), [qureg[0]])
+        for index, action in gate.term:
+            fe.apply({"X": XGate, "Y": YGate, "Z": ZGate}[action](), [qureg[index]])
 
 
 # --- meta sections --------------------------------------------------------------------------
@@ -892,11 +1001,16 @@ class _Frontend:
         return stage
 
     def apply(self, gate, *regs, controls=(), control_state=None):
-        if isinstance(gate, TimeEvolution):
+        if isinstance(gate, (TimeEvolution, QubitOperatorGate)):
+            # these gates' | operator rewrites them first (onto the qubits they act on)
             if len(regs) != 1:
                 raise TypeError("Only one qubit or qureg allowed.")
-            gate, reg = gate.commands_for(list(regs[0]))
-            regs = (reg,)
+            for g, reg in gate.commands_for(list(regs[0])):
+                self._apply_one(g, [reg], controls, control_state)
+            return
+        self._apply_one(gate, regs, controls, control_state)
+
+    def _apply_one(self, gate, regs, controls, control_state):
         cmd = Command(gate, [list(r) for r in regs])
         if len(controls):
             cmd.add_control_qubits(list(controls), control_state)
@@ -1063,7 +1177,8 @@ _FIXED_GATES = {
     "H": HGate, "X": XGate, "Y": YGate, "Z": ZGate, "S": SGate, "T": TGate, "SqrtX": SqrtXGate,
     "Swap": SwapGate, "SqrtSwap": SqrtSwapGate, "QFT": QFTGate, "Barrier": BarrierGate,
     "AddQuantum": AddQuantumGate, "SubtractQuantum": SubtractQuantumGate,
-    "ComparatorQuantum": ComparatorQuantumGate,
+    "ComparatorQuantum": ComparatorQuantumGate, "MultiplyQuantum": MultiplyQuantumGate,
+    "DivideQuantum": DivideQuantumGate, "Entangle": EntangleGate,
 }
 _PARAM_GATES = {
     "Rx": Rx, "Ry": Ry, "Rz": Rz, "Rxx": Rxx, "Ryy": Ryy, "Rzz": Rzz, "Ph": Ph, "R": R,
@@ -1081,6 +1196,8 @@ def make_gate(name, params=()):
         return get_inverse(_FIXED_GATES[name[:-3]]())
     if name == "Matrix":
         return MatrixGate([[complex(re, im) for re, im in row] for row in params])
+    if name == "QubitOperator":  # params = [[[index, "X"], ...], [re, im]]
+        return QubitOperatorGate([tuple(lp) for lp in params[0]], complex(*params[1]))
     if name == "TimeEvolution":  # params = [time, [[term, coeff], ...]], term = [[index, "X"], ...]
         return TimeEvolution(params[0], [(tuple(tuple(lp) for lp in term), c) for term, c in params[1]])
     if name in _PARAM_GATES:

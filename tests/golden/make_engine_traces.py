@@ -190,6 +190,27 @@ def prog_time_evolution():
     return ops
 
 
+def prog_more_gates():
+    """Multiply/Divide quantum and their inverses, Entangle, QubitOperator gates (identity,
+    single-qubit and multi-qubit terms, a control, and two different adjacent Pauli strings on the
+    same qubits, which the reference's optimizer cancels because QubitOperator gates compare equal)."""
+    ops = [["alloc", 10], g("X", [[0]]), g("X", [[3]]), g("H", [[1]])]
+    ops += [g("MultiplyQuantum", [[0, 1], [2, 3], [4, 5, 6, 7, 8]])]
+    ops += [g("MultiplyQuantumdag", [[0, 1], [2, 3], [4, 5, 6, 7, 8]], ctrls=[9])]
+    ops += [g("DivideQuantum", [[0, 1, 2], [3, 4, 5], [6, 7, 8]])]
+    ops += [g("DivideQuantumdag", [[0, 1, 2], [3, 4, 5], [6, 7, 8]])]
+    ops += [g("Entangle", [[0, 1, 2, 3, 4]]), g("Entangle", [[5, 6, 7]], ctrls=[9])]
+    ops += [g("QubitOperator", [list(range(6))], [[[0, "X"], [2, "Z"]], [0.0, 1.0]])]
+    ops += [g("QubitOperator", [list(range(6))], [[[2, "Y"]], [math.cos(0.3), math.sin(0.3)]])]
+    ops += [g("QubitOperator", [[3, 4]], [[], [-1.0, 0.0]])]
+    ops += [g("QubitOperator", [[4, 5, 6]], [[[0, "X"], [2, "Y"]], [0.0, -1.0]], ctrls=[9])]
+    ops += [g("QubitOperator", [[7, 8]], [[[0, "X"], [1, "X"]], [1.0, 0.0]])]
+    ops += [g("QubitOperator", [[7, 8]], [[[0, "Z"], [1, "Z"]], [1.0, 0.0]])]
+    ops += [["dagger"], g("QubitOperator", [[1, 3, 5]], [[[0, "Y"], [1, "Z"], [2, "X"]], [0.0, 1.0]]), ["dagger_end"]]
+    ops += [["measure", q] for q in range(10)] + [["dealloc", q] for q in range(10)]
+    return ops
+
+
 # --- seeded random programs -----------------------------------------------------------------
 ANGLE_POOL = [0.0, PI / 4, PI / 2, PI, 2 * PI, 3 * PI, 4 * PI - 1e-13, 4 * PI, -PI / 2, 1e-13]
 
@@ -274,6 +295,7 @@ PROGRAMS = {
     "layers6": prog_layers(),
     "identity_multiqubit": prog_identity_multiqubit(),
     "time_evolution": prog_time_evolution(),
+    "more_gates": prog_more_gates(),
 }
 for _s in range(12):
     PROGRAMS[f"random{_s}"] = prog_random(100 + _s, n=4 + _s % 3)
@@ -288,7 +310,7 @@ def run_reference(ops):
     from projectq.libs import math as qmath
     from projectq.meta import Compute, Control, Dagger, Uncompute
     from projectq.ops import (
-        QFT, Barrier, H, MatrixGate, Measure, Ph, QubitOperator, R, Rx, Rxx, Ry, Ryy, Rz, Rzz, S, Sdag, SqrtSwap,
+        QFT, Barrier, Entangle, H, MatrixGate, Measure, Ph, QubitOperator, R, Rx, Rxx, Ry, Ryy, Rz, Rzz, S, Sdag, SqrtSwap,
         SqrtX, Swap, T, Tdag, TimeEvolution, X, Y, Z, get_inverse,
     )
     from projectq.types import Qureg
@@ -302,7 +324,9 @@ def run_reference(ops):
              "SqrtXdag": get_inverse(SqrtX), "Swap": Swap, "SqrtSwap": SqrtSwap, "QFT": QFT,
              "QFTdag": get_inverse(QFT), "Barrier": Barrier, "AddQuantum": qmath.AddQuantum,
              "AddQuantumdag": get_inverse(qmath.AddQuantum), "SubtractQuantum": qmath.SubtractQuantum,
-             "ComparatorQuantum": qmath.ComparatorQuantum}
+             "ComparatorQuantum": qmath.ComparatorQuantum, "MultiplyQuantum": qmath.MultiplyQuantum,
+             "MultiplyQuantumdag": get_inverse(qmath.MultiplyQuantum), "DivideQuantum": qmath.DivideQuantum,
+             "DivideQuantumdag": get_inverse(qmath.DivideQuantum), "Entangle": Entangle}
     param = {"Rx": Rx, "Ry": Ry, "Rz": Rz, "Rxx": Rxx, "Ryy": Ryy, "Rzz": Rzz, "Ph": Ph, "R": R,
              "AddConstant": qmath.AddConstant, "SubConstant": qmath.SubConstant,
              "AddConstantModN": qmath.AddConstantModN, "SubConstantModN": qmath.SubConstantModN,
@@ -315,6 +339,8 @@ def run_reference(ops):
             return fixed[name]
         if name == "Matrix":
             return MatrixGate([[complex(re, im) for re, im in row] for row in params])
+        if name == "QubitOperator":
+            return QubitOperator(tuple(tuple(lp) for lp in params[0]), complex(*params[1]))
         if name == "TimeEvolution":
             ham = QubitOperator()
             for term, c in params[1]:

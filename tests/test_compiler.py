@@ -102,7 +102,9 @@ def test_qft_expansion_order():
     assert gates == [([2], []), ([2], [1]), ([2], [0]), ([1], []), ([1], [0]), ([0], [])]
 
 
-@pytest.mark.parametrize("name", ["layers6", "qft", "math", "compute_uncompute", "time_evolution", "random3"])
+@pytest.mark.parametrize(
+    "name", ["layers6", "qft", "math", "compute_uncompute", "time_evolution", "more_gates", "random3"]
+)
 def test_default_list_runs_on_the_oracle_like_the_recorded_calls(name):
     """The compiled program and the reference's recorded calls, each run on the oracle with the
     same seed, measure the same outcomes and end in the same state (one amplitude once every
@@ -173,7 +175,7 @@ def test_default_list_on_the_gpu(precision):
     from oracle.cpu_sim import OracleSimulator
     from projectq_amd import Simulator
 
-    for name in ("layers6", "qft", "math", "time_evolution", "random5"):
+    for name in ("layers6", "qft", "math", "time_evolution", "more_gates", "random5"):
         ops = [op for op in TRACES[name]["ops"] if op[0] not in ("dealloc", "measure")]
         calls = [c for c in TRACES[name]["calls"] if c[0] not in ("deallocate_qubit", "measure_qubits")]
         h = Simulator(3, precision=precision)
