@@ -455,7 +455,23 @@ __device__ __forceinline__ void reg_op(uint32_t code, R *vr, R *vi, const amp_t<
 #undef QS_R2C
 #undef QS_R2B
 
+// One register op: global controls are uniform over the block (a branch); tile-bit controls are
+// per lane (the predicated form).
 template <typename R, int RB>
+__device__ __forceinline__ void reg_apply(const RegOp *__restrict__ op, uint64_t base, uint32_t lb, R *vr, R *vi,
+                                          const amp_t<R> *__restrict__ mats) {
+  const uint64_t gc = op->gcmask;
+  const uint32_t lc = op->lcmask;
+  if ((base & gc) == gc) {
+    if (lc == 0) reg_op<R, RB, false>(op->code, vr, vi, mats + op->mat, true);
+    else reg_op<R, RB, true>(op->code, vr, vi, mats + op->mat, (lb & lc) == lc);
+  }
+}
+
+// TURN ops per turn of the op loop.  With 2 the register allocator moves the amplitudes between
+// register sets once per turn rather than once per op (worth about 2% on the pass kernel), at
+// the price of a second copy of the op dispatch; kernels off the hot path use 1.
+template <typename R, int RB, int TURN>
 __device__ __forceinline__ void reg_stage(amp_t<R> *__restrict__ tile, uint32_t t, uint64_t base,
                                           const TileStage *__restrict__ st,
                                           const RegOp *__restrict__ rops,
@@ -482,28 +498,10 @@ __device__ __forceinline__ void reg_stage(amp_t<R> *__restrict__ tile, uint32_t 
       vr[i] = a.x;
       vi[i] = a.y;
     }
-    // Two ops per turn of the loop: the register allocator copies the amplitudes between two
-    // register sets at the loop's back edge, once per turn rather than once per op.
-    for (uint32_t o = ob; o < oe; o += 2) {
-      // global controls are uniform over the block (a branch); tile-bit controls are per lane
-      // (a predicate inside the op)
-      {
-        const RegOp *op = rops + o;
-        const uint64_t gc = op->gcmask;
-        const uint32_t lc = op->lcmask;
-        if ((base & gc) == gc) {
-          if (lc == 0) reg_op<R, RB, false>(op->code, vr, vi, mats + op->mat, true);
-          else reg_op<R, RB, true>(op->code, vr, vi, mats + op->mat, (lb & lc) == lc);
-        }
-      }
-      if (o + 1 < oe) {
-        const RegOp *op = rops + o + 1;
-        const uint64_t gc = op->gcmask;
-        const uint32_t lc = op->lcmask;
-        if ((base & gc) == gc) {
-          if (lc == 0) reg_op<R, RB, false>(op->code, vr, vi, mats + op->mat, true);
-          else reg_op<R, RB, true>(op->code, vr, vi, mats + op->mat, (lb & lc) == lc);
-        }
+    for (uint32_t o = ob; o < oe; o += TURN) {
+      reg_apply<R, RB>(rops + o, base, lb, vr, vi, mats);
+      if constexpr (TURN == 2) {
+        if (o + 1 < oe) reg_apply<R, RB>(rops + o + 1, base, lb, vr, vi, mats);
       }
     }
     // Recompute the LDS addresses for the write-back instead of keeping 2^RB of them live across
@@ -599,14 +597,14 @@ __device__ __forceinline__ uint64_t tile_base(uint64_t tix, const TilePass &P) {
 constexpr int kStreamChunk = 8;
 
 // The stages of a pass on one tile resident in LDS, a workgroup barrier after each.
-template <typename R, int RB>
+template <typename R, int RB, int TURN>
 __device__ __forceinline__ void run_stages(amp_t<R> *__restrict__ tile, uint32_t t, uint64_t base, uint32_t n_stages,
                                            const TileStage *__restrict__ stages, const RegOp *__restrict__ rops,
                                            const TileOp *__restrict__ tops, const amp_t<R> *__restrict__ mats) {
   for (uint32_t s = 0; s < n_stages; ++s) {
     const TileStage *st = stages + s;
     if (st->kind == kStageRegisters) {
-      reg_stage<R, RB>(tile, t, base, st, rops, mats);
+      reg_stage<R, RB, TURN>(tile, t, base, st, rops, mats);
     } else {
       const TileOp *op = tops + st->begin;
       const uint64_t gc = op->gcmask;
@@ -656,7 +654,7 @@ __device__ __forceinline__ void tile_body(amp_t<R> *__restrict__ psi, uint64_t t
     }
   }
   __syncthreads();
-  run_stages<R, RB>(tile, t, base, P.n_stages, stages, rops, tops, mats);
+  run_stages<R, RB, 1>(tile, t, base, P.n_stages, stages, rops, tops, mats);
   for (uint32_t l0 = threadIdx.x * VN; l0 < tsize; l0 += kStreamChunk * kStep) {
     const uint32_t s0 = lds_address(tile) + slot_bytes<R>(l0);  // l0 shares no bit with j * kStep
     V v[kStreamChunk];
@@ -753,7 +751,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
   };
   if (next < n_tiles) prefetch(nbase);
   while (true) {
-    run_stages<R, RB>(tile, t, base, P.n_stages, stages, rops, tops, mats);
+    run_stages<R, RB, 2>(tile, t, base, P.n_stages, stages, rops, tops, mats);
     uint32_t ls = l0;
     asm volatile("" : "+v"(ls));
 #pragma unroll
