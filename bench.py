@@ -285,7 +285,8 @@ def default_engine_list(args, n=20, layers=20):
     ProjectQ program and compiled by the default engine list (projectq_amd.compiler) in front of
     the GPU simulator: once as the reference runs it by default (complex128, run() after every
     gate) and once with gate_fusion=True into complex64 window fusion with the learned planner.
-    Times include the host-side compiler."""
+    Then the headline circuit (all its layers) through the same front end into complex64 window
+    fusion.  Times include the host-side compiler."""
     from projectq_amd import Simulator, circuits
     from projectq_amd import compiler as C
 
@@ -317,6 +318,29 @@ def default_engine_list(args, n=20, layers=20):
                       "backend_gate_calls": calls["apply_controlled_gate"],
                       "outcomes": "".join("1" if eng.results[q] else "0" for q in range(n))}
         del eng, sim
+
+    # the headline circuit (32 qubits, 100 layers, 16 CNOTs, measure all) through the same front end
+    big_n, big_layers = args.qubits, args.steps
+    ops = circuits.random_layers_program(big_n, big_layers, CNOTS_PER_LAYER, args.seed)
+    big_gates = sum(1 for op in ops if op[0] == "gate")
+    sim = Simulator(args.seed, precision="c64", fusion="window", planner="learned", capacity_qubits=big_n)
+    backend = C.SimulatorEngine(sim, gate_fusion=True)
+    t0 = time.perf_counter()
+    eng = C.MainEngine(backend)
+    C.run_program(ops[:-big_n], backend, engine=eng)  # all but the measurements, then a flush
+    t_host = time.perf_counter() - t0  # compiler + planner + (asynchronous) pass launches
+    sim.synchronize()
+    t_gates = time.perf_counter() - t0
+    for q in range(big_n):
+        eng.measure(q)
+    eng.flush()
+    sim.synchronize()
+    t_all = time.perf_counter() - t0
+    out[f"c64_window_learned_{big_n}q_{big_layers}_layers"] = {
+        "gates": big_gates, "compile_plan_launch_s": round(t_host, 3), "gates_s": round(t_gates, 3),
+        "gates_per_s": round(big_gates / t_gates, 1), "with_measure_all_s": round(t_all, 3),
+        "plan": sim.last_plan()}
+    del eng, backend, sim
     log(f"default engine list: {out}")
     return out
 

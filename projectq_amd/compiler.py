@@ -1205,15 +1205,16 @@ def make_gate(name, params=()):
     raise ValueError(f"unknown gate {name!r}")
 
 
-def run_program(ops, backend, engine_list=None):
+def run_program(ops, backend, engine_list=None, engine=None, flush=True):
     """Run a program of the op language on `backend` (a SimulatorEngine) through the default
-    engine list; returns the MainEngine (its `results` hold the measurement outcomes).
+    engine list (or through `engine`, a MainEngine to continue), flushing at the end unless
+    `flush` is false; returns the MainEngine (its `results` hold the measurement outcomes).
 
         ["alloc", k]  ["gate", name, params, regs, ctrls, state]  ["measure", q]  ["dealloc", q]
         ["flush"]  ["compute"]  ["compute_end"]  ["uncompute"]  ["dagger"]  ["dagger_end"]
         ["control", ctrls, state]  ["control_end"]
     """
-    eng = MainEngine(backend, engine_list)
+    eng = MainEngine(backend, engine_list) if engine is None else engine
     open_sections = []
     for op in ops:
         kind = op[0]
@@ -1244,5 +1245,6 @@ def run_program(ops, backend, engine_list=None):
             open_sections.pop().__exit__(None, None, None)
         else:
             raise ValueError(f"unknown op {kind!r}")
-    eng.flush()
+    if flush:
+        eng.flush()
     return eng
