@@ -261,3 +261,42 @@ def test_pipelined_tile_pass_26q_c64():
         bits = [(int(idx) >> q) & 1 for q in range(n)]
         a, b = ref.get_amplitude(bits, list(range(n))), tile.get_amplitude(bits, list(range(n)))
         assert abs(a - b) <= 1e-4 * 2 ** (-n / 2)  # typical |amplitude| is 2^(-n/2)
+
+
+def test_tile_pass_2p11_tiles_25q_c128():
+    """25 qubits, complex128, tiles of 2^11 amplitudes (one 16-byte amplitude per streamed vector,
+    eight per thread): 2^14 tiles on at most 8192 workgroups, so workgroups loop over several
+    tiles.  Checked against the unfused complex128 run of the same gates."""
+    from projectq_amd import Simulator
+
+    n = 25
+    print(f"tile pass 2^11 c128 n={n}", flush=True)
+    gates = [c for c in circuits.random_layers(n, 3, n // 2, 43, measure=False) if c[0] == "gate"]
+
+    def run(tile_shape=None, **opts):
+        sim = Simulator(5, capacity_qubits=n, **opts)
+        if tile_shape:
+            sim.set_tile_params(*tile_shape)
+        for q in range(n):
+            sim.allocate_qubit(q)
+        window = opts.get("fusion") == "window"
+        for _, m, t, c in gates:
+            sim.apply_controlled_gate(m, list(t), list(c))
+            if not window:
+                sim.run()
+        sim.run()
+        sim.synchronize()
+        return sim
+
+    ref = run(precision="c128", fusion="reference")
+    for shape in ((11, 4, 4), (11, 4, 5)):
+        tile = run(shape, precision="c128", fusion="window", planner="tile")
+        assert tile.last_plan()["tile_bits"] == 11
+        inf = 1.0 - abs(ref.overlap(tile)) ** 2 / (ref.overlap(ref).real * tile.overlap(tile).real)
+        assert abs(inf) <= 1e-12, inf
+        rng = np.random.default_rng(5)
+        for idx in rng.integers(0, 2**n, size=64):
+            bits = [(int(idx) >> q) & 1 for q in range(n)]
+            a, b = ref.get_amplitude(bits, list(range(n))), tile.get_amplitude(bits, list(range(n)))
+            assert abs(a - b) <= 1e-12 * 2 ** (-n / 2)
+        del tile
