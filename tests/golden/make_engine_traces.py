@@ -296,6 +296,10 @@ PROGRAMS = {
     "identity_multiqubit": prog_identity_multiqubit(),
     "time_evolution": prog_time_evolution(),
     "more_gates": prog_more_gates(),
+    "negative_control_merge": [["alloc", 3], g("H", [[0]]), g("H", [[1]]), g("Rx", [[1]], [0.3], [0], "0"),
+                               g("Rx", [[1]], [0.4], [0], "0"), g("X", [[2]], [], [0], "0"), g("X", [[2]], [], [0], "1"),
+                               ["measure", 0], ["measure", 1], ["measure", 2], ["dealloc", 0], ["dealloc", 1],
+                               ["dealloc", 2]],
 }
 for _s in range(12):
     PROGRAMS[f"random{_s}"] = prog_random(100 + _s, n=4 + _s % 3)
