@@ -55,18 +55,27 @@ std::vector<PPass> schedule_tiles(const std::vector<PGate> &gates, const TileSha
   const int G = (int)gates.size();
   const unsigned n = shape.n, t = std::min(shape.t, shape.n);
   const unsigned L0 = std::min(shape.Lmin, t);
-  // predecessors: the last earlier gate touching each of this gate's positions
+  // predecessors.  A target position is written, a control position only read: gates that
+  // share a control (and touch it nowhere else) commute, so a gate depends on the last writer of
+  // each position it touches and, for its targets, on every reader since that writer.
   std::vector<std::vector<int>> preds(G);
   {
-    std::vector<int> last(64, -1);
+    std::vector<int> writer(64, -1);
+    std::vector<std::vector<int>> readers(64);
     for (int i = 0; i < G; ++i) {
-      uint64_t touch = gates[i].tmask() | gates[i].cmask;
+      const uint64_t tm = gates[i].tmask(), cm = gates[i].cmask;
       std::set<int> ps;
-      for (uint32_t p = 0; p < 64; ++p)
-        if ((touch >> p) & 1) {
-          if (last[p] >= 0) ps.insert(last[p]);
-          last[p] = i;
+      for (uint32_t p = 0; p < 64; ++p) {
+        if (!(((tm | cm) >> p) & 1)) continue;
+        if (writer[p] >= 0) ps.insert(writer[p]);
+        if ((tm >> p) & 1) {
+          ps.insert(readers[p].begin(), readers[p].end());
+          readers[p].clear();
+          writer[p] = i;
+        } else {
+          readers[p].push_back(i);
         }
+      }
       preds[i].assign(ps.begin(), ps.end());
     }
   }
