@@ -50,20 +50,33 @@ std::vector<uint32_t> choose_by_score(const std::vector<SlotCandidate> &cands, u
   return out;
 }
 
+// A gate whose matrix is diagonal commutes with controls on its targets and with other diagonal
+// gates there, so for dependencies its targets are read, like controls.
+static bool diagonal_gate(const PGate &g) {
+  const size_t d = size_t(1) << g.tpos.size();
+  if (g.m.size() != d * d) return false;
+  for (size_t r = 0; r < d; ++r)
+    for (size_t c = 0; c < d; ++c)
+      if (r != c && g.m[r * d + c] != std::complex<double>(0.0, 0.0)) return false;
+  return true;
+}
+
 std::vector<PPass> schedule_tiles(const std::vector<PGate> &gates, const TileShape &shape,
                                   const SlotChooser &choose, unsigned lookahead) {
   const int G = (int)gates.size();
   const unsigned n = shape.n, t = std::min(shape.t, shape.n);
   const unsigned L0 = std::min(shape.Lmin, t);
-  // predecessors.  A target position is written, a control position only read: gates that
-  // share a control (and touch it nowhere else) commute, so a gate depends on the last writer of
-  // each position it touches and, for its targets, on every reader since that writer.
+  // predecessors.  A non-diagonal gate writes its target positions; controls and the targets of
+  // diagonal gates are only read.  Readers of a position commute with each other, so a gate
+  // depends on the last writer of each position it touches and, where it writes, on every reader
+  // since that writer.
   std::vector<std::vector<int>> preds(G);
   {
     std::vector<int> writer(64, -1);
     std::vector<std::vector<int>> readers(64);
     for (int i = 0; i < G; ++i) {
-      const uint64_t tm = gates[i].tmask(), cm = gates[i].cmask;
+      const bool diag = diagonal_gate(gates[i]);
+      const uint64_t tm = diag ? 0 : gates[i].tmask(), cm = gates[i].cmask | (diag ? gates[i].tmask() : 0);
       std::set<int> ps;
       for (uint32_t p = 0; p < 64; ++p) {
         if (!(((tm | cm) >> p) & 1)) continue;

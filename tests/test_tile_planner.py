@@ -290,3 +290,48 @@ def test_lane_controlled_mix_gpu(precision):
     got = sim.cheat_numpy()[1]
     tol = 1e-12 if precision == "c128" else 2e-5
     assert np.max(np.abs(got - want)) <= tol
+
+
+def diagonal_mix(n, n_gates, seed):
+    """Diagonal gates (Rz, phase, controlled phase, Rzz) interleaved with gates that do not commute
+    with them (H, Rx, X, CNOT with the diagonal gates' qubits as targets): the scheduler may move
+    diagonal gates and controls past each other but never past a write to their qubits."""
+    rng = np.random.default_rng(seed)
+    cmds = circuits.allocate(n)
+    for _ in range(n_gates):
+        r = rng.random()
+        a = float(rng.uniform(0, 4 * np.pi))
+        q = [int(x) for x in rng.choice(n, size=3, replace=False)]
+        if r < 0.2:
+            cmds.append(commands.gate(G.Rz(a), [q[0]]))
+        elif r < 0.35:
+            cmds.append(commands.gate(G.R(a), [q[0]], [q[1]]))
+        elif r < 0.45:
+            cmds.append(commands.gate(G.Rzz(a), [q[0], q[1]], [q[2]] if rng.random() < 0.3 else []))
+        elif r < 0.6:
+            cmds.append(commands.gate(G.X, [q[0]], [q[1]]))
+        elif r < 0.75:
+            cmds.append(commands.gate(G.H, [q[0]]))
+        elif r < 0.9:
+            cmds.append(commands.gate(G.Rx(a), [q[0]], [q[1]] if rng.random() < 0.5 else []))
+        else:
+            cmds.append(commands.gate(G.Z, [q[0]], [q[1], q[2]]))
+    return cmds
+
+
+@pytest.mark.parametrize("c64", [True, False])
+@pytest.mark.parametrize("n,t,lmin,rb", [(14, 10, 4, 4), (15, 12, 4, 4), (13, 12, 5, 5)])
+def test_diagonal_gates_reorder_correctly(emu, c64, n, t, lmin, rb):
+    cmds = diagonal_mix(n, 500, 100 + n)
+    want = oracle_state(cmds, n)
+    start = np.zeros(2**n, dtype=np.complex128)
+    start[:] = 1 / np.sqrt(2**n)  # a uniform superposition: every phase is visible
+    o = OracleSimulator(1)
+    for q in range(n):
+        o.allocate_qubit(q)
+    o.set_wavefunction(start, list(range(n)))
+    commands.execute(o, [c for c in cmds if c[0] == "gate"], gate_fusion=False)
+    want = o.cheat_numpy()[1]
+    got, npass = run_emu(emu, n, t, lmin, cmds, start, rb, c64)
+    assert np.max(np.abs(got - want)) <= 1e-12
+    assert npass < 200
