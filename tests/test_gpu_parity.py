@@ -300,3 +300,30 @@ def test_tile_pass_2p11_tiles_25q_c128():
             a, b = ref.get_amplitude(bits, list(range(n))), tile.get_amplitude(bits, list(range(n)))
             assert abs(a - b) <= 1e-12 * 2 ** (-n / 2)
         del tile
+
+
+def test_headline_shape_32q_round_trip():
+    """The measured configuration at its full size (32 qubits, complex64, window fusion, learned
+    planner, so the pipelined tile pass on 2^20 tiles): three layers of the random-layer circuit
+    followed by their inverse return the state to |0...0>.  A size-independent check of the
+    kernel on the real shape, where no CPU reference fits."""
+    from projectq_amd import Simulator
+
+    n = 32
+    print(f"32-qubit round trip", flush=True)
+    gates = [c for c in circuits.random_layers(n, 3, 16, 77, measure=False) if c[0] == "gate"]
+    inverse = [(k, G.dagger(m), t, c) for k, m, t, c in reversed(gates)]
+    sim = Simulator(9, precision="c64", fusion="window", planner="learned", capacity_qubits=n)
+    for q in range(n):
+        sim.allocate_qubit(q)
+    for _, m, t, c in gates + inverse:
+        sim.apply_controlled_gate(m, list(t), list(c))
+    sim.run()
+    plan = sim.last_plan()
+    assert plan["kind"] == "tile" and plan["tile_bits"] == 12
+    zero = [False] * n
+    p0 = sim.get_probability(zero, list(range(n)))
+    assert p0 >= 1 - 1e-4, p0
+    total = sim.get_probability([False], [0]) + sim.get_probability([True], [0])
+    assert abs(total - 1.0) <= 1e-4
+    del sim
