@@ -824,7 +824,29 @@ __global__ __launch_bounds__(kBlock) void k_batch_energy(const amp_t<R> *__restr
     const amp_t<R> a = st[i];
     const double ar = re(a), ai = im(a);
     double ur = 0.0, ui = 0.0;
-    for (int t = 0; t < nt; ++t) {
+    // four partner loads in flight before they are used: the loop is bound by load latency
+    int t = 0;
+    for (; t + 4 <= nt; t += 4) {
+      amp_t<R> b[4];
+      uint64_t j[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        j[u] = i ^ tm[t + u].x;
+        b[u] = st[j[u]];
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const PauliTerm pt = tm[t + u];
+        double br = re(b[u]), bi = im(b[u]);
+        if (__popcll(j[u] & pt.z) & 1) {
+          br = -br;
+          bi = -bi;
+        }
+        ur += br * pt.c_re - bi * pt.c_im;
+        ui += br * pt.c_im + bi * pt.c_re;
+      }
+    }
+    for (; t < nt; ++t) {
       const PauliTerm pt = tm[t];
       const uint64_t j = i ^ pt.x;
       const amp_t<R> b = st[j];
