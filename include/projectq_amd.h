@@ -186,6 +186,10 @@ int qs_reset_stats(qs_simulator *sim);
 /* <a|b> over two simulators on the same GPU holding the same qubits at the same positions
  * (precisions may differ); accumulated in FP64.  result = (re, im). */
 int qs_overlap(qs_simulator *a, qs_simulator *b, double *result);
+/* Element-wise comparison of the states of two simulators on the same GPU holding the same qubits
+ * at the same positions (precisions may differ): *max_abs_diff = max_i |a_i - b_i| (FP64) and
+ * *n_unequal = the number of amplitudes whose stored values differ. */
+int qs_compare_states(qs_simulator *a, qs_simulator *b, double *max_abs_diff, uint64_t *n_unequal);
 
 /* ---- planner controls and the learned cost model (extensions, no counterpart in the reference) --
  * Tile shape of QS_PLANNER_TILE: tile bits (6..13; 12 for complex128), register bits (4 or 5) and

@@ -874,6 +874,7 @@ API int qs_load_state(qs_simulator *s, const char *path) {
       ok = std::fread(v.data(), 8, na, f) == na;
       for (uint64_t i = 0; ok && i < na; ++i) amps[i] = cplx(v[2 * i], v[2 * i + 1]);
     }
+    ok = ok && std::fgetc(f) == EOF;  // no trailing bytes
     std::fclose(f);
     if (!ok) throw std::invalid_argument("load_state: malformed state file");
     if (s->o.pos.empty()) {
@@ -909,6 +910,21 @@ API int qs_overlap(qs_simulator *a, qs_simulator *b, double *out) {
     for (size_t i = 0; i < a->o.psi.size(); ++i) acc += std::conj(a->o.psi[i]) * b->o.psi[i];
     out[0] = acc.real();
     out[1] = acc.imag();
+  });
+}
+API int qs_compare_states(qs_simulator *a, qs_simulator *b, double *max_abs_diff, uint64_t *n_unequal) {
+  return guarded([&] {
+    a->o.run();
+    b->o.run();
+    if (a->o.pos != b->o.pos) throw std::invalid_argument("compare: different qubits or positions");
+    double m = 0.0;
+    uint64_t c = 0;
+    for (size_t i = 0; i < a->o.psi.size(); ++i) {
+      m = std::max(m, std::abs(a->o.psi[i] - b->o.psi[i]));
+      c += a->o.psi[i] != b->o.psi[i];
+    }
+    *max_abs_diff = m;
+    *n_unequal = c;
   });
 }
 

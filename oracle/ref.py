@@ -12,6 +12,11 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 _cache = {}
 
 
+def available(variant="intrin"):
+    """Whether the build exists, without loading it."""
+    return bool(glob.glob(os.path.join(HERE, "_ref", variant, "_cppsim*.so")))
+
+
 def load(variant="intrin"):
     if variant in _cache:
         return _cache[variant]

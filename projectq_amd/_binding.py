@@ -104,6 +104,7 @@ _SIGNATURES = {
     ),
     "qs_reset_stats": (ctypes.c_int, [ctypes.c_void_p]),
     "qs_overlap": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, _f64p]),
+    "qs_compare_states": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, _f64p, ctypes.POINTER(ctypes.c_uint64)]),
     "qs_set_tile_params": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32]),
     "qs_set_recording": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     "qs_take_records": (
@@ -540,6 +541,16 @@ class CAbiSimulator:
         out = np.zeros(2, dtype=np.float64)
         self._check(self._lib.qs_overlap(self._handle, other._handle, _ptr(out, ctypes.c_double)))
         return complex(out[0], out[1])
+
+    def compare(self, other):
+        """(max |a_i - b_i|, number of amplitudes whose stored values differ) between this
+        simulator's state and `other`'s, computed on the GPU (same qubits at the same positions)."""
+        if other._lib is not self._lib:
+            raise ValueError("compare needs two simulators of the same library")
+        mx = ctypes.c_double()
+        cnt = ctypes.c_uint64()
+        self._check(self._lib.qs_compare_states(self._handle, other._handle, ctypes.byref(mx), ctypes.byref(cnt)))
+        return mx.value, cnt.value
 
     # ------------------------------------------------------------------ planner controls
     def set_tile_params(self, tile_bits, reg_bits, low_bits):

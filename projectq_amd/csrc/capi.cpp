@@ -255,6 +255,13 @@ API int qs_overlap(qs_simulator *a, qs_simulator *b, double *result) {
   });
 }
 
+API int qs_compare_states(qs_simulator *a, qs_simulator *b, double *max_abs_diff, uint64_t *n_unequal) {
+  return guarded(a, [&] {
+    if (a->device != b->device) throw qs::ValueError("compare: the simulators live on different GPUs");
+    a->engine.compare(b->engine, max_abs_diff, n_unequal);
+  });
+}
+
 struct qs_batch {
   qs::Batch batch;
   int device;

@@ -413,27 +413,42 @@ PassFeatures Engine::block_features(const Block &b) const {
   return block_pass_features((unsigned)b.qubits.size(), (unsigned)b.ctrl.size(), lo, n_, c64_);
 }
 
+// Plans the whole window before any pass runs; if planning fails the window is left queued and
+// the state untouched.
 void Engine::flush_window() {
   if (window_.empty()) return;
   for (const Gate &g : window_)
     if (g.ids.empty() || g.ids.size() > 5) throw ValueError("Gates with more than 5 qubits are not supported!");
-  std::vector<Gate> gates;
-  gates.swap(window_);
-  last_plan_ = PlanInfo{};
+  std::vector<Block> blocks;
+  std::vector<TileProgram> progs;
+  unsigned rb = tile_rb_;
+  PlanInfo info;
+  plan_window(window_, &blocks, &progs, &rb, &info);
+  window_.clear();
+  last_plan_ = info;
+  if (info.kind == 0) {
+    for (const Block &b : blocks) exec_block(b);
+  } else {
+    for (const TileProgram &p : progs) exec_tile_program(p, rb);
+  }
+}
+
+void Engine::plan_window(const std::vector<Gate> &gates, std::vector<Block> *blocks_out,
+                         std::vector<TileProgram> *progs_out, unsigned *rb_out, PlanInfo *info) {
+  *info = PlanInfo{};
   if (planner_kind_ == 0 || (planner_kind_ == 1 && !model_)) {
     // greedy max-5 blocks: the reference's rule, and the learned planner's fallback
-    std::vector<Block> blocks = greedy_blocks(gates);
-    last_plan_.kind = 0;
-    last_plan_.passes = (unsigned)blocks.size();
-    for (const Block &b : blocks) exec_block(b);
+    *blocks_out = greedy_blocks(gates);
+    info->kind = 0;
+    info->passes = (unsigned)blocks_out->size();
     return;
   }
   const std::vector<PGate> pg = resolve(gates);
   if (planner_kind_ == 2) {
     unsigned t;
-    std::vector<TileProgram> progs = plan_tiles(pg, tile_tmax_, tile_rb_, tile_lmin_, &t);
-    last_plan_ = PlanInfo{1, tile_tmax_, tile_rb_, tile_lmin_, (unsigned)progs.size(), 1, 0.0};
-    for (const TileProgram &p : progs) exec_tile_program(p, tile_rb_);
+    *progs_out = plan_tiles(pg, tile_tmax_, tile_rb_, tile_lmin_, &t);
+    *rb_out = tile_rb_;
+    *info = PlanInfo{1, tile_tmax_, tile_rb_, tile_lmin_, (unsigned)progs_out->size(), 1, 0.0};
     return;
   }
   // learned planner: predict the time of every pass of every candidate plan, run the cheapest
@@ -472,21 +487,22 @@ void Engine::flush_window() {
       best = c;
     }
   }
-  last_plan_.candidates = (unsigned)rows_of.size();
-  last_plan_.predicted_ms = best_ms;
+  info->candidates = (unsigned)rows_of.size();
+  info->predicted_ms = best_ms;
   if (best == cands.size()) {
-    last_plan_.kind = 0;
-    last_plan_.passes = (unsigned)blocks.size();
-    for (const Block &b : blocks) exec_block(b);
+    info->kind = 0;
+    info->passes = (unsigned)blocks.size();
+    *blocks_out = std::move(blocks);
     return;
   }
   const Cand &c = cands[best];
-  last_plan_.kind = 1;
-  last_plan_.tmax = c.tmax;
-  last_plan_.rb = c.rb;
-  last_plan_.lmin = c.lmin;
-  last_plan_.passes = (unsigned)plans[best].size();
-  for (const TileProgram &p : plans[best]) exec_tile_program(p, c.rb);
+  info->kind = 1;
+  info->tmax = c.tmax;
+  info->rb = c.rb;
+  info->lmin = c.lmin;
+  info->passes = (unsigned)plans[best].size();
+  *rb_out = c.rb;
+  *progs_out = std::move(plans[best]);
 }
 
 void Engine::begin_pass(hipEvent_t *e0, hipEvent_t *e1) {
@@ -662,6 +678,24 @@ cplx Engine::overlap(Engine &other) {
   double im;
   double re = finish(dev::kReduceGrid, 2, &im);
   return cplx(re, im);
+}
+
+void Engine::compare(Engine &other, double *max_abs_diff, uint64_t *n_unequal) {
+  run();
+  other.run();
+  other.sync();
+  if (other.pos_ != pos_) throw ValueError("compare: the simulators hold different qubits or positions");
+  dev::compare(psi_, c64_, other.psi_, other.c64_, amps(), d_partials_, stream_);
+  std::vector<double> part(2 * dev::kReduceGrid);
+  QS_HIP(hipMemcpyAsync(part.data(), d_partials_, part.size() * sizeof(double), hipMemcpyDeviceToHost, stream_));
+  sync();
+  double m = 0.0, c = 0.0;
+  for (int b = 0; b < dev::kReduceGrid; ++b) {
+    m = std::max(m, part[2 * b]);
+    c += part[2 * b + 1];
+  }
+  *max_abs_diff = m;
+  *n_unequal = (uint64_t)c;
 }
 
 // ------------------------------------------------------------------------------ emulate_math
@@ -1133,6 +1167,15 @@ void Engine::load_state(const char *path) {
     throw ValueError("load_state: not a state file of this simulator");
   if (h.n_qubits > 40 || h.n_amps != (uint64_t(1) << h.n_qubits) || h.precision > 1)
     throw ValueError("load_state: corrupt header");
+  // the whole file must be there before the qubit map or the state is touched
+  {
+    const long here = std::ftell(file.f);
+    if (here < 0 || std::fseek(file.f, 0, SEEK_END) != 0) throw RuntimeError("load_state: cannot size the file");
+    const long size = std::ftell(file.f);
+    if (size < 0 || std::fseek(file.f, here, SEEK_SET) != 0) throw RuntimeError("load_state: cannot size the file");
+    const uint64_t want = sizeof(StateHeader) + uint64_t(8) * h.n_qubits + h.n_amps * (h.precision == 1 ? 8u : 16u);
+    if ((uint64_t)size != want) throw ValueError("load_state: the file is truncated or has trailing bytes");
+  }
   std::map<unsigned, unsigned> map;
   std::vector<bool> seen(h.n_qubits, false);
   for (uint32_t i = 0; i < h.n_qubits; ++i) {

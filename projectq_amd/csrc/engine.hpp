@@ -89,6 +89,9 @@ class Engine {
   void get_stats(uint64_t *passes, double *bytes, double *gpu_ms, uint64_t *timed);
   void reset_stats();
   cplx overlap(Engine &other);
+  // max |a_i - b_i| and the number of amplitudes whose stored values differ (same qubits at the
+  // same positions required)
+  void compare(Engine &other, double *max_abs_diff, uint64_t *n_unequal);
   unsigned num_qubits() const { return n_; }
   unsigned position(unsigned id) const { return pos_.at(id); }
   hipStream_t stream() const { return stream_; }
@@ -117,6 +120,8 @@ class Engine {
   void math_apply(dev::MathArgs args, const int32_t *d_table, bool known_bijective);
   double masked_norm(uint64_t mask, uint64_t val);
   void flush_window();
+  void plan_window(const std::vector<Gate> &gates, std::vector<Block> *blocks, std::vector<TileProgram> *progs,
+                   unsigned *rb, PlanInfo *info);
   unsigned tile_tmax_ = 13, tile_rb_ = 5;  // tile shape limits (PROJECTQ_AMD_TILE="t:rb" overrides)
   unsigned tile_bits(unsigned tmax) const;
   std::vector<Block> greedy_blocks(const std::vector<Gate> &gates) const;
@@ -170,6 +175,8 @@ class Engine {
   char *d_arena_ = nullptr, *h_arena_ = nullptr;
   size_t arena_off_ = 0;
   static constexpr size_t kArenaBytes = size_t(8) << 20;
+  // one tile pass uploads four arrays, each rounded up to 256 bytes
+  static_assert(kArenaBytes >= kPassUploadBudget + 4 * 256, "a tile pass program must fit the upload arena");
 };
 
 }  // namespace qs

@@ -134,6 +134,11 @@ void math_accumulate(const amp_t<R> *src, amp_t<R> *dst, uint64_t n_amps, const 
 void overlap(const void *a, bool a_c64, const void *b, bool b_c64, uint64_t n_amps,
              double *partials, hipStream_t s);
 
+// partials[2*b] <- block max of |a_i - b_i|, partials[2*b + 1] <- block count of i with
+// a_i != b_i (stored values compared exactly); either operand c128 or c64
+void compare(const void *a, bool a_c64, const void *b, bool b_c64, uint64_t n_amps, double *partials,
+             hipStream_t s);
+
 // dst[i] = src[i] converted between complex128 and complex64 (round to nearest on narrowing)
 template <typename S, typename D>
 void convert_amps(const amp_t<S> *src, amp_t<D> *dst, uint64_t n, hipStream_t s);

@@ -137,8 +137,9 @@ std::vector<PPass> schedule_tiles(const std::vector<PGate> &gates, const TileSha
     pass.L = L;
     for (uint32_t p : high)
       if (p >= L) pass.high.push_back(p);
-    // sweep in queue order
+    // sweep in queue order, up to the upload budget of one pass
     const int sweep_end = std::min(G, first + 8192);
+    size_t upload = 0;
     for (int i = first; i < sweep_end; ++i) {
       if (done[i]) continue;
       if ((gates[i].tmask() & ~tmask) != 0) continue;
@@ -149,6 +150,9 @@ std::vector<PPass> schedule_tiles(const std::vector<PGate> &gates, const TileSha
           break;
         }
       if (!ready) continue;
+      const size_t cost = gate_upload_bytes(gates[i]);
+      if (!pass.gates.empty() && upload + cost > kPassUploadBudget) break;
+      upload += cost;
       done[i] = 1;
       pass.gates.push_back(i);
     }

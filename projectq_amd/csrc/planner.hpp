@@ -59,6 +59,16 @@ using SlotChooser =
 // that position in between) into one matrix product.
 std::vector<PGate> merge_single_qubit_runs(const std::vector<PGate> &gates);
 
+// Upper bound on the device bytes one tile pass uploads (stage, op and matrix records): a pass
+// is closed before a gate that would take it past this budget, so every pass program fits the
+// engine's upload arena whatever the gate mix.
+constexpr size_t kPassUploadBudget = size_t(4) << 20;
+// Worst-case upload bytes of one gate: its complex128 matrix plus an LDS op and its own stage.
+inline size_t gate_upload_bytes(const PGate &g) {
+  return sizeof(std::complex<double>) * (size_t(1) << (2 * g.tpos.size())) + sizeof(dev::TileOp) +
+         sizeof(dev::TileStage) + sizeof(dev::RegOp);
+}
+
 // Tile scheduling: repeatedly pick a tile (forced: the targets of the first unscheduled gate;
 // the other high slots from `choose`, default = highest lookahead score) and take, in queue
 // order, every unscheduled gate whose predecessors are scheduled and whose targets are in it.

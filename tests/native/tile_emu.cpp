@@ -4,6 +4,7 @@
 // replays every tile pass on the CPU with the kernel's own index arithmetic, checking each
 // global and LDS index against its bounds.  Lets the CPU test-suite check that the planner's
 // passes are in bounds and that their result equals the unfused gate sequence.
+#include <algorithm>
 #include <complex>
 #include <cstdint>
 #include <cstring>
@@ -297,6 +298,39 @@ extern "C" int emu_op_kinds(unsigned n, unsigned t, unsigned lmin, unsigned rb, 
       }
     }
     return 0;
+  } catch (const std::exception &e) {
+    std::strncpy(err, e.what(), err_len - 1);
+    return 2;
+  }
+}
+
+// Largest device upload of one pass program (stage, register-op, LDS-op and complex128 matrix
+// arrays, each rounded up to 256 bytes as the engine's upload arena does) and the pass count.
+extern "C" int emu_max_pass_upload(unsigned n, unsigned t, unsigned lmin, unsigned rb, int c64, int n_gates,
+                                   const uint32_t *k, const uint32_t *tpos, const uint64_t *cmask,
+                                   const double *mats, uint64_t *max_bytes, int *n_passes, char *err, int err_len) {
+  try {
+    auto gates = qs::merge_single_qubit_runs(read_gates(n_gates, k, tpos, cmask, mats));
+    auto passes = qs::schedule_tiles(gates, qs::TileShape{n, t, lmin});
+    auto up = [](size_t b) { return (b + 255) & ~size_t(255); };
+    uint64_t most = 0;
+    size_t covered = 0;
+    for (auto &ps : passes) {
+      qs::TileProgram prog;
+      qs::build_tile_pass(gates, ps, std::min(t, n), rb, c64 != 0, prog);
+      const uint64_t b = up(prog.stages.size() * sizeof(qs::dev::TileStage)) +
+                         up(prog.rops.size() * sizeof(qs::dev::RegOp)) +
+                         up(prog.tops.size() * sizeof(qs::dev::TileOp)) + up(prog.mats.size() * 16);
+      most = std::max(most, b);
+      covered += ps.gates.size();
+    }
+    check(covered == gates.size(), "every gate scheduled exactly once");
+    *max_bytes = most;
+    *n_passes = (int)passes.size();
+    return 0;
+  } catch (const Fail &f) {
+    std::strncpy(err, f.what.c_str(), err_len - 1);
+    return 1;
   } catch (const std::exception &e) {
     std::strncpy(err, e.what(), err_len - 1);
     return 2;

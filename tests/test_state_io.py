@@ -48,7 +48,7 @@ def test_oracle_rejects_malformed_state_files(tmp_path):
     path = str(tmp_path / "state.qs")
     o.save_state(path)
     data = open(path, "rb").read()
-    for bad in (b"XXXXXXXX" + data[8:], data[:-16], data[:40]):
+    for bad in (b"XXXXXXXX" + data[8:], data[:-16], data[:40], data + b"\0" * 16):
         p = str(tmp_path / "bad.qs")
         with open(p, "wb") as f:
             f.write(bad)
@@ -100,6 +100,31 @@ def test_state_file_round_trip_large_gpu(tmp_path):
     for idx in rng.integers(0, 2**n, size=32):
         bits = [(int(idx) >> q) & 1 for q in range(n)]
         assert g.get_amplitude(bits, list(range(n))) == h.get_amplitude(bits, list(range(n)))
+
+
+@pytest.mark.gpu
+def test_malformed_state_file_leaves_the_gpu_simulator_unchanged(tmp_path):
+    """A truncated or padded checkpoint is rejected (ValueError) before the qubit map or the
+    state change."""
+    from projectq_amd import Simulator
+
+    o = prepared_oracle()
+    path = str(tmp_path / "state.qs")
+    o.save_state(path)
+    data = open(path, "rb").read()
+    h = Simulator(1)
+    commands.execute(h, circuits.random_layers(10, 3, 4, 9, measure=False), gate_fusion=False)
+    h.allocate_qubit(100)  # the file's qubit ids, at other positions
+    h_map, h_state = h.qubit_map(), h.cheat_numpy()[1]
+    assert set(h_map) == set(o.qubit_map())
+    for bad in (data[:-16], data[:-1], data + b"\0" * 16):
+        p = str(tmp_path / "bad.qs")
+        with open(p, "wb") as f:
+            f.write(bad)
+        with pytest.raises(ValueError):
+            h.load_state(p)
+        assert h.qubit_map() == h_map
+        assert np.array_equal(h.cheat_numpy()[1], h_state)
 
 
 @pytest.mark.gpu

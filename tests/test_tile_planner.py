@@ -335,3 +335,66 @@ def test_diagonal_gates_reorder_correctly(emu, c64, n, t, lmin, rb):
     got, npass = run_emu(emu, n, t, lmin, cmds, start, rb, c64)
     assert np.max(np.abs(got - want)) <= 1e-12
     assert npass < 200
+
+
+def _gate_arrays(cmds):
+    ks, tpos, cm, mats = [], [], [], []
+    for c in cmds:
+        if c[0] != "gate":
+            continue
+        _, m, targets, controls = c
+        ks.append(len(targets))
+        tpos.extend(targets)
+        mask = 0
+        for q in controls:
+            mask |= 1 << q
+        cm.append(mask)
+        mats.append(np.asarray(m, dtype=np.complex128).reshape(-1))
+    return (np.asarray(ks, dtype=np.uint32), np.asarray(tpos, dtype=np.uint32), np.asarray(cm, dtype=np.uint64),
+            np.ascontiguousarray(np.concatenate(mats)))
+
+
+def dense_gates(n, count, k, low, seed):
+    """`count` random dense k-qubit unitaries on qubits below `low` (all inside one tile)."""
+    rng = np.random.default_rng(seed)
+    cmds = circuits.allocate(n)
+    for _ in range(count):
+        picks = rng.choice(low, size=k, replace=False)
+        a = rng.normal(size=(2**k, 2**k)) + 1j * rng.normal(size=(2**k, 2**k))
+        u, _ = np.linalg.qr(a)
+        cmds.append(commands.gate(u, [int(q) for q in picks]))
+    return cmds
+
+
+ARENA_BYTES = 8 << 20  # the engine's upload arena (engine.hpp kArenaBytes)
+
+
+@pytest.mark.parametrize("k,count", [(4, 3000), (5, 3000)])
+def test_pass_upload_fits_the_arena(emu, k, count):
+    """Thousands of dense 4- and 5-qubit gates that all fit one 12-bit tile: the scheduler closes
+    each pass at its upload budget, so every pass program fits the engine's upload arena (a single
+    pass would need 11.7 MiB / 46.9 MiB of matrices)."""
+    cmds = dense_gates(20, count, k, 12, 30 + k)
+    ks, tp, cm, mf = _gate_arrays(cmds)
+    most, npass = ctypes.c_uint64(), ctypes.c_int()
+    err = ctypes.create_string_buffer(256)
+    emu.emu_max_pass_upload.restype = ctypes.c_int
+    rc = emu.emu_max_pass_upload(
+        ctypes.c_uint(20), ctypes.c_uint(12), ctypes.c_uint(5), ctypes.c_uint(4), ctypes.c_int(0), ctypes.c_int(len(ks)),
+        ks.ctypes.data_as(ctypes.c_void_p), tp.ctypes.data_as(ctypes.c_void_p), cm.ctypes.data_as(ctypes.c_void_p),
+        mf.ctypes.data_as(ctypes.c_void_p), ctypes.byref(most), ctypes.byref(npass), err, ctypes.c_int(256))
+    assert rc == 0, err.value.decode()
+    total = count * 16 * 4**k
+    assert most.value <= ARENA_BYTES
+    assert npass.value >= total // ARENA_BYTES + 1
+
+
+def test_split_passes_match_unfused(emu):
+    """Passes closed by the upload budget (1100 dense 5-qubit gates in one tile: at least five
+    passes) still give the unfused result."""
+    n = 12
+    cmds = dense_gates(n, 1100, 5, n, 77)
+    want = oracle_state(cmds, n)
+    got, npass = run_emu(emu, n, 12, 5, cmds, start_state(n), 4, False)
+    assert np.max(np.abs(got - want)) <= 1e-12
+    assert npass >= 5
