@@ -171,7 +171,7 @@ def test_headline_32q_depth100_infidelity():
     print(f"c64 learned vs c128 tile: {a:.3e}", flush=True)
     assert a <= 1e-6
     greedy = run("c64", "greedy")
-    assert greedy.last_plan()["kind"] == "greedy"
+    assert greedy.last_plan()["kind"] == "greedy_blocks"
     b = infid(ref, greedy)
     c = infid(learned, greedy)
     print(f"c64 greedy vs c128 tile: {b:.3e}; vs c64 learned: {c:.3e}", flush=True)

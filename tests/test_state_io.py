@@ -113,7 +113,7 @@ def test_malformed_state_file_leaves_the_gpu_simulator_unchanged(tmp_path):
     o.save_state(path)
     data = open(path, "rb").read()
     h = Simulator(1)
-    commands.execute(h, circuits.random_layers(10, 3, 4, 9, measure=False), gate_fusion=False)
+    commands.execute(h, circuits.random_layers(9, 3, 4, 9, measure=False), gate_fusion=False)
     h.allocate_qubit(100)  # the file's qubit ids, at other positions
     h_map, h_state = h.qubit_map(), h.cheat_numpy()[1]
     assert set(h_map) == set(o.qubit_map())
