@@ -116,6 +116,19 @@ int qs_emulate_math_op(qs_simulator *sim, int opcode, const int64_t *params, siz
 int qs_get_expectation_value(qs_simulator *sim, const uint32_t *term_idx, const uint8_t *term_ops,
                              const size_t *term_offsets, size_t n_terms, const double *coeffs,
                              const uint32_t *ids, size_t n_ids, double *result);
+/* No counterpart in the reference: qs_get_expectation_value for callers that cannot allocate
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).  Same
+ * kernels and the same result as qs_get_expectation_value, on a caller-owned state of 2^n_qubits
+ * amplitudes at `device_state` on the current GPU (complex128 for QS_C128, complex64 for QS_C64,
+ * e.g. from qs_device_state); local index l of a string is the qubit at bit position
+ * positions[l].  Allocates no device or host memory: the strings, partial sums and per-string
+ * results use a fixed 224 KiB device array per GPU (held until the process ends).  At most 4096
+ * strings per call (QS_ERR_VALUE beyond).  Synchronous on the null stream; calls that overlap on
+ * one GPU share that array and race. */
+int qs_expectation_value_noalloc(const void *device_state, uint32_t n_qubits, int precision,
+                                 const uint32_t *term_idx, const uint8_t *term_ops, const size_t *term_offsets,
+                                 size_t n_terms, const double *coeffs, const uint32_t *positions,
+                                 size_t n_positions, double *result);
 /* simulator.hpp:324-350; coeffs are complex (interleaved) */
 int qs_apply_qubit_operator(qs_simulator *sim, const uint32_t *term_idx, const uint8_t *term_ops,
                             const size_t *term_offsets, size_t n_terms, const double *coeffs,

@@ -828,6 +828,10 @@ API int qs_qubit_map(qs_simulator *s, uint32_t *ids, uint32_t *positions, size_t
 API int qs_device_state(qs_simulator *, void **, uint64_t *, int *) {
   return guarded([&] { throw std::runtime_error("the oracle keeps its state in host memory"); });
 }
+API int qs_expectation_value_noalloc(const void *, uint32_t, int, const uint32_t *, const uint8_t *, const size_t *,
+                                    size_t, const double *, const uint32_t *, size_t, double *) {
+  return guarded([&] { throw std::runtime_error("the oracle keeps its state in host memory"); });
+}
 API int qs_set_wavefunction_device(qs_simulator *, const void *, uint64_t, int, const uint32_t *, size_t) {
   return guarded([&] { throw std::runtime_error("the oracle keeps its state in host memory"); });
 }

@@ -104,6 +104,11 @@ _SIGNATURES = {
     ),
     "qs_reset_stats": (ctypes.c_int, [ctypes.c_void_p]),
     "qs_overlap": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, _f64p]),
+    "qs_expectation_value_noalloc": (
+        ctypes.c_int,
+        [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_int, _u32p, _u8p, _szp, ctypes.c_size_t, _f64p, _u32p,
+         ctypes.c_size_t, _f64p],
+    ),
     "qs_compare_states": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, _f64p, ctypes.POINTER(ctypes.c_uint64)]),
     "qs_set_tile_params": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32]),
     "qs_set_recording": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
@@ -243,6 +248,31 @@ def _pauli_terms(terms, complex_coeffs):
         np.asarray(offsets, dtype=np.uintp),
         np.asarray(coeffs, dtype=np.float64),
     )
+
+
+def expectation_value_noalloc(lib, state, positions, terms):
+    """qs_expectation_value_noalloc: sum_t c_t Re<psi|P_t|psi> on a device state (a DeviceState or
+    any 1-d contiguous complex64/complex128 array with `__cuda_array_interface__` on the current
+    GPU) with no allocation; local index l of a string is the qubit at bit position positions[l].
+    `terms` in the backend's format [(list[(local index, 'X'|'Y'|'Z')], real coefficient)]."""
+    iface = state.__cuda_array_interface__
+    n_amps, typestr = int(iface["shape"][0]), iface["typestr"]
+    if typestr not in ("<c8", "<c16"):
+        raise ValueError("expectation_value_noalloc: complex64 or complex128 amplitudes")
+    n_qubits = n_amps.bit_length() - 1
+    if n_amps != 1 << n_qubits:
+        raise ValueError("expectation_value_noalloc: the state must hold 2^n amplitudes")
+    idx, ops, offs, coeffs = _pauli_terms(terms, complex_coeffs=False)
+    pos = _u32(positions)
+    out = ctypes.c_double()
+    rc = lib.qs_expectation_value_noalloc(
+        ctypes.c_void_p(int(iface["data"][0])), n_qubits, 1 if typestr == "<c8" else 0, _ptr(idx, ctypes.c_uint32),
+        _ptr(ops, ctypes.c_uint8), _ptr(offs, ctypes.c_size_t), len(offs) - 1, _ptr(coeffs, ctypes.c_double),
+        _ptr(pos, ctypes.c_uint32), len(pos), ctypes.byref(out))
+    if rc != QS_OK:
+        msg = (lib.qs_last_error() or b"").decode(errors="replace")
+        raise {QS_ERR_VALUE: ValueError, QS_ERR_MEMORY: MemoryError}.get(rc, RuntimeError)(msg)
+    return out.value
 
 
 def _regs(quregs):

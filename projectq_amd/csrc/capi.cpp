@@ -11,6 +11,7 @@
 #include "batch.hpp"
 #include "engine.hpp"
 #include "mlp.hpp"
+#include "noalloc.hpp"
 
 struct qs_simulator {
   qs::Engine engine;
@@ -155,6 +156,16 @@ API int qs_get_expectation_value(qs_simulator *s, const uint32_t *term_idx, cons
   return guarded(s, [&] {
     *result = s->engine.get_expectation_value(terms_of(term_idx, term_ops, term_offsets, n_terms),
                                               std::vector<double>(coeffs, coeffs + n_terms), ids_of(ids, n_ids));
+  });
+}
+API int qs_expectation_value_noalloc(const void *device_state, uint32_t n_qubits, int precision,
+                                    const uint32_t *term_idx, const uint8_t *term_ops, const size_t *term_offsets,
+                                    size_t n_terms, const double *coeffs, const uint32_t *positions,
+                                    size_t n_positions, double *result) {
+  return guarded(nullptr, [&] {
+    if (precision != QS_C128 && precision != QS_C64) throw qs::ValueError("precision must be QS_C128 or QS_C64");
+    *result = qs::expectation_noalloc(device_state, n_qubits, precision == QS_C64, term_idx, term_ops, term_offsets,
+                                      n_terms, coeffs, positions, n_positions);
   });
 }
 API int qs_apply_qubit_operator(qs_simulator *s, const uint32_t *term_idx, const uint8_t *term_ops,

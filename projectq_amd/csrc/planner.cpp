@@ -446,24 +446,25 @@ PassFeatures block_pass_features(unsigned k, unsigned n_ctrl, unsigned min_targe
   return F;
 }
 
+void PauliEncoder::add(uint32_t pos, char op) {
+  if (pos >= 64) throw std::invalid_argument("Pauli string position out of range");
+  const uint64_t q = uint64_t(1) << pos;
+  const double sgn = (x & q) ? -1.0 : 1.0;
+  switch (op) {
+    case 'X': x ^= q; break;
+    case 'Z': omega *= sgn; z ^= q; break;
+    case 'Y': omega *= std::complex<double>(0.0, sgn); x ^= q; z ^= q; break;
+    default: throw std::invalid_argument("Pauli operator must be X, Y or Z");
+  }
+}
+
 void encode_pauli(const std::vector<std::pair<uint32_t, char>> &ops, uint64_t *x, uint64_t *z,
                   std::complex<double> *omega) {
-  uint64_t X = 0, Z = 0;
-  std::complex<double> w(1.0, 0.0);
-  for (const auto &op : ops) {
-    if (op.first >= 64) throw std::invalid_argument("Pauli string position out of range");
-    const uint64_t q = uint64_t(1) << op.first;
-    const double sgn = (X & q) ? -1.0 : 1.0;
-    switch (op.second) {
-      case 'X': X ^= q; break;
-      case 'Z': w *= sgn; Z ^= q; break;
-      case 'Y': w *= std::complex<double>(0.0, sgn); X ^= q; Z ^= q; break;
-      default: throw std::invalid_argument("Pauli operator must be X, Y or Z");
-    }
-  }
-  *x = X;
-  *z = Z;
-  *omega = w;
+  PauliEncoder e;
+  for (const auto &op : ops) e.add(op.first, op.second);
+  *x = e.x;
+  *z = e.z;
+  *omega = e.omega;
 }
 
 }  // namespace qs

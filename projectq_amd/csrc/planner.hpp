@@ -94,6 +94,12 @@ void build_tile_pass(const std::vector<PGate> &gates, const PPass &pass, unsigne
 // omega *= i (-1)^{x_q}, x ^= q, z ^= q.  Then (P psi)_i = omega (-1)^{popcount((i^x)&z)} psi_{i^x}.
 void encode_pauli(const std::vector<std::pair<uint32_t, char>> &ops, uint64_t *x, uint64_t *z,
                   std::complex<double> *omega);
+// The same encoding fed one op at a time (no allocation).
+struct PauliEncoder {
+  uint64_t x = 0, z = 0;
+  std::complex<double> omega{1.0, 0.0};
+  void add(uint32_t pos, char op);
+};
 
 // ---- cost-model features of one pass (inputs of the learned planner's MLP) -----------------
 constexpr int kPassFeatures = 24;
