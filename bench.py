@@ -451,11 +451,14 @@ def main():
     if not args.no_c128_window and args.precision == "c64":
         ref_sim, ref_wall, _, ref_stats = gpu_window_run(args, cmds, n, "c128", "c128 window")
         ov = ref_sim.overlap(sim)
+        n_ref, n_c64 = ref_sim.overlap(ref_sim).real, sim.overlap(sim).real
         infid = 1.0 - abs(ov) ** 2
         total_p = ref_sim.get_probability([False], [0]) + ref_sim.get_probability([True], [0])
         log(f"infidelity of the complex64 state against complex128: {infid:.3e}")
         accuracy = {
             "infidelity_vs_c128": infid,
+            "infidelity_vs_c128_normalized": 1.0 - abs(ov) ** 2 / (n_ref * n_c64),
+            "c64_norm_minus_1": n_c64 - 1.0,
             "c128_window": {
                 "gates_per_s": round(n_gates / ref_wall, 2),
                 "ms_per_step": round(1e3 * ref_wall / args.steps, 3),
@@ -520,6 +523,8 @@ def main():
     }
     if accuracy:
         out["infidelity_vs_c128"] = accuracy["infidelity_vs_c128"]
+        out["infidelity_vs_c128_normalized"] = accuracy["infidelity_vs_c128_normalized"]
+        out["c64_norm_minus_1"] = accuracy["c64_norm_minus_1"]
         out["c128_window"] = accuracy["c128_window"]
         out["speedup_vs_c128_window"] = round(value / accuracy["c128_window"]["gates_per_s"], 3)
 

@@ -31,6 +31,23 @@ std::vector<uint32_t> bits_of(uint64_t mask) {
   return out;
 }
 
+// ||M^H M - I||_F of a d x d matrix, in FP64; with `as_fp32` the entries are first rounded to
+// FP32 as the complex64 kernels use them.  Bounds the relative norm change M makes to any state.
+double unitarity_defect(const std::vector<cplx> &m, size_t d, bool as_fp32) {
+  auto at = [&](size_t r, size_t c) {
+    const cplx v = m[r * d + c];
+    return as_fp32 ? cplx((double)(float)v.real(), (double)(float)v.imag()) : v;
+  };
+  double f = 0.0;
+  for (size_t i = 0; i < d; ++i)
+    for (size_t j = 0; j < d; ++j) {
+      cplx acc(0.0, 0.0);
+      for (size_t r = 0; r < d; ++r) acc += std::conj(at(r, i)) * at(r, j);
+      f += std::norm(acc - (i == j ? cplx(1.0, 0.0) : cplx(0.0, 0.0)));
+    }
+  return std::sqrt(f);
+}
+
 }  // namespace
 
 Engine::Engine(uint32_t seed, int precision, int fusion_mode, int planner, unsigned capacity_hint)
@@ -246,6 +263,7 @@ void Engine::deallocate_qubit(unsigned id) {
     if (kv.second > p) kv.second--;
   pos_.erase(id);
   n_--;
+  norm_changed();
 }
 
 // ------------------------------------------------------------------------------ measurement
@@ -312,6 +330,7 @@ std::vector<uint8_t> Engine::measure_qubits(const std::vector<unsigned> &ids) {
     using R = decltype(tag);
     dev::collapse<R>((amp_t<R> *)psi_, N, mask, val, scale, stream_);
   });
+  norm_changed();
   return res;
 }
 
@@ -322,7 +341,21 @@ void Engine::apply_controlled_gate(Gate g) {
   check_known(g.ctrl, "apply_controlled_gate");
   if (g.m.size() != (size_t(1) << (2 * g.ids.size())))
     throw ValueError("apply_controlled_gate: matrix size does not match the number of qubits");
+  // the complex64 norm guard must know whether a non-unitary gate is among the gates of the next
+  // execution, and how far their FP32 matrices may move the norm
+  bool nonunitary = false;
+  double drift = 0.0;
+  if (c64_) {
+    const size_t d = size_t(1) << g.ids.size();
+    nonunitary = unitarity_defect(g.m, d, false) > 1e-9;
+    drift = unitarity_defect(g.m, d, true);
+  }
+  auto queued = [&]() {
+    queued_nonunitary_ |= nonunitary;
+    queued_drift_ += drift;
+  };
   if (fusion_mode_ == 1) {
+    queued();
     window_.push_back(std::move(g));
     return;
   }
@@ -331,11 +364,14 @@ void Engine::apply_controlled_gate(Gate g) {
   const unsigned now = trial.num_qubits(), before = queue_.num_qubits();
   if (now >= 4 && now <= 5) {
     queue_ = std::move(trial);
+    queued();
     run();
   } else if (now > 5 || (size_t)(now - g.ids.size()) > (size_t)before) {
     run();
     queue_.insert(g);
+    queued();
   } else {
+    queued();
     queue_ = std::move(trial);
   }
 }
@@ -348,8 +384,39 @@ void Engine::run() {
   }
   if (queue_.empty()) return;
   Block b = queue_.fuse();
+  if (b.qubits.empty() || b.qubits.size() > 5) throw ValueError("Gates with more than 5 qubits are not supported!");
+  before_unitary_passes();
   exec_block(b);  // throws for > 5 qubits before the queue is cleared, like the reference
   queue_.clear();
+  after_passes();
+}
+
+void Engine::before_unitary_passes() {
+  if (!c64_ || queued_nonunitary_ || norm_known_) return;
+  norm_ref_ = masked_norm(0, 0);
+  norm_known_ = true;
+  drift_bound_ = 0.0;
+}
+
+void Engine::after_passes() {
+  if (!c64_) return;
+  const double drift = queued_drift_;
+  queued_drift_ = 0.0;
+  if (queued_nonunitary_) {
+    queued_nonunitary_ = false;
+    norm_changed();
+    return;
+  }
+  drift_bound_ += drift;
+  if (drift_bound_ < kNormDriftBudget) return;
+  const double cur = masked_norm(0, 0);
+  if (cur > 0.0 && norm_ref_ > 0.0) {
+    dispatch([&](auto tag) {
+      using R = decltype(tag);
+      dev::scale_masked<R>((amp_t<R> *)psi_, amps(), 0, std::sqrt(norm_ref_ / cur), 0.0, stream_);
+    });
+  }
+  drift_bound_ = 0.0;
 }
 
 // Greedy planner over a window of queued gates: the reference's block rule (min 4 / max 5,
@@ -426,11 +493,13 @@ void Engine::flush_window() {
   plan_window(window_, &blocks, &progs, &rb, &info);
   window_.clear();
   last_plan_ = info;
+  before_unitary_passes();
   if (info.kind == 0) {
     for (const Block &b : blocks) exec_block(b);
   } else {
     for (const TileProgram &p : progs) exec_tile_program(p, rb);
   }
+  after_passes();
 }
 
 void Engine::plan_window(const std::vector<Gate> &gates, std::vector<Block> *blocks_out,
@@ -743,6 +812,7 @@ void Engine::math_apply(dev::MathArgs args, const int32_t *d_table, bool known_b
   sync();
   if (h_flags_[0]) throw RuntimeError("emulate_math: a register value does not fit a C++ int");
   std::swap(psi_, buf_[0]);
+  norm_changed();
 }
 
 // simulator.hpp:224-290 for the opcodes of include/projectq_amd.h
@@ -880,6 +950,7 @@ void Engine::apply_qubit_operator(const std::vector<PauliString> &terms, const s
                       0.0, false, 0, d_partials_, stream_);
   });
   std::swap(psi_, buf_[0]);
+  norm_changed();
 }
 
 // simulator.hpp:386-438: scaled, truncated Taylor series of exp(-i H t) on the controlled
@@ -934,6 +1005,7 @@ void Engine::emulate_time_evolution(const std::vector<PauliString> &terms, const
     QS_HIP(hipMemcpyAsync(psi_, out, N * amp_bytes(), hipMemcpyDeviceToDevice, stream_));
   }
   sync();
+  norm_changed();
 }
 
 // ------------------------------------------------------------------------------ queries
@@ -998,6 +1070,7 @@ void Engine::set_wavefunction(const double *wf, size_t n_amps, const std::vector
     QS_HIP(hipMemcpyAsync(psi_, wf, n_amps * 16, hipMemcpyHostToDevice, stream_));
     sync();
   }
+  norm_changed();
 }
 
 // simulator.hpp:456-485
@@ -1022,6 +1095,7 @@ void Engine::collapse_wavefunction(const std::vector<unsigned> &ids, const std::
     using R = decltype(tag);
     dev::collapse<R>((amp_t<R> *)psi_, amps(), mask, val, scale, stream_);
   });
+  norm_changed();
 }
 
 // simulator.hpp:529-532
@@ -1061,6 +1135,7 @@ void Engine::device_state(void **ptr, uint64_t *n_amps, int *precision) {
   *ptr = psi_;
   *n_amps = amps();
   *precision = c64_ ? 1 : 0;
+  norm_changed();  // the caller may write through the pointer
 }
 
 void Engine::qubit_map(uint32_t *ids, uint32_t *positions, size_t capacity, size_t *n_qubits) {
@@ -1236,6 +1311,7 @@ void Engine::load_state(const char *path) {
     k ^= 1;
   }
   sync();
+  norm_changed();
 }
 
 void Engine::set_wavefunction_device(const void *src, uint64_t n_amps, int src_precision,
@@ -1262,6 +1338,7 @@ void Engine::set_wavefunction_device(const void *src, uint64_t n_amps, int src_p
                                      n_amps, stream_);
   }
   sync();
+  norm_changed();
 }
 
 }  // namespace qs

@@ -120,6 +120,24 @@ class Engine {
   void math_apply(dev::MathArgs args, const int32_t *d_table, bool known_bijective);
   double masked_norm(uint64_t mask, uint64_t val);
   void flush_window();
+  // complex64 norm guard: FP32-rounded gate matrices are unitary only to ~1e-8 (the FP32 H scales
+  // the norm by 1 - 3.4e-8), so over thousands of gates the state's norm drifts from its exact
+  // value.  While only unitary gates run, the FP64 norm taken before them is the exact target.
+  // Each gate adds a bound on its norm change, ||M32^H M32 - I||_F of its FP32-rounded matrix; when
+  // the bound accumulated since the last check reaches kNormDriftBudget, the state is rescaled to
+  // the target after the execution (one norm pass and one scaling pass).
+  static constexpr double kNormDriftBudget = 2.5e-7;
+  double norm_ref_ = 1.0;      // norm^2 of the state before the unitary gates since the last check
+  bool norm_known_ = true;     // norm_ref_ is valid (the empty state |> has norm 1)
+  double drift_bound_ = 0.0;   // bound on the relative norm change of the gates since the check
+  bool queued_nonunitary_ = false;  // a gate queued since the last execution is not unitary
+  double queued_drift_ = 0.0;       // drift bound of the gates queued since the last execution
+  void norm_changed() {
+    norm_known_ = false;
+    drift_bound_ = 0.0;
+  }
+  void before_unitary_passes();
+  void after_passes();
   void plan_window(const std::vector<Gate> &gates, std::vector<Block> *blocks, std::vector<TileProgram> *progs,
                    unsigned *rb, PlanInfo *info);
   unsigned tile_tmax_ = 13, tile_rb_ = 5;  // tile shape limits (PROJECTQ_AMD_TILE="t:rb" overrides)

@@ -168,8 +168,9 @@ def test_headline_32q_depth100_infidelity():
     assert plan["kind"] == "tile" and plan["tile_bits"] == 12 and plan["reg_bits"] == 4
     ref = run("c128", "tile")
     a = infid(ref, learned)
-    print(f"c64 learned vs c128 tile: {a:.3e}", flush=True)
-    assert a <= 1e-6
+    raw = 1.0 - abs(ref.overlap(learned)) ** 2  # norm error included
+    print(f"c64 learned vs c128 tile: {a:.3e} ({raw:.3e} with the norm error)", flush=True)
+    assert a <= 1e-6 and raw <= 1e-6
     greedy = run("c64", "greedy")
     assert greedy.last_plan()["kind"] == "greedy_blocks"
     b = infid(ref, greedy)

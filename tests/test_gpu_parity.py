@@ -327,3 +327,38 @@ def test_headline_shape_32q_round_trip():
     total = sim.get_probability([False], [0]) + sim.get_probability([True], [0])
     assert abs(total - 1.0) <= 1e-4
     del sim
+
+
+@pytest.mark.parametrize("fusion", ["reference", "window"])
+def test_c64_norm_guard_on_long_unitary_sequences(fusion):
+    """complex64 gate matrices are rounded to FP32 and so unitary only to ~1e-8 (the FP32 H scales
+    the norm by about 1 - 3.4e-8): 4000 H gates would move the norm by ~1.4e-4.  The engine
+    rescales complex64 states to the FP64 norm taken before a run of unitary gates, so the norm
+    stays at 1 to float accuracy; a non-unitary gate switches the guard off until the norm is
+    measured again (its norm change is kept, as in complex128)."""
+    n = 10
+    h = Simulator(1, precision="c64", fusion=fusion, planner="tile")
+    for q in range(n):
+        h.allocate_qubit(q)
+    for i in range(4000):
+        h.apply_controlled_gate(G.H, [i % n], [])
+        if fusion == "window" and i % 250 == 249:
+            h.run()
+    h.run()
+    norm = h.overlap(h).real
+    assert abs(norm - 1.0) <= 1e-6, norm
+    # a non-unitary diagonal: the norm it removes stays removed
+    shrink = np.diag([1.0, 0.5]).astype(complex)
+    ref = Simulator(1, precision="c128")
+    for sim in (h, ref):
+        if sim is ref:
+            for q in range(n):
+                sim.allocate_qubit(q)
+            for i in range(4000):
+                sim.apply_controlled_gate(G.H, [i % n], [])
+                sim.run()
+        for i in range(70):
+            sim.apply_controlled_gate(shrink, [i % n], [])
+            sim.apply_controlled_gate(G.H, [i % n], [])
+        sim.run()
+    assert abs(h.overlap(h).real - ref.overlap(ref).real) <= 1e-6 * ref.overlap(ref).real
