@@ -391,6 +391,20 @@ def calibrate_copy(nbytes=1 << 31):
     log(f"calibration copy: {nbytes} bytes read and written")
 
 
+def stream_ceiling(sim, repeats=3):
+    """The practical streaming rates a pass kernel is measured against, on the headline state:
+    collapse_wavefunction([], []) runs the norm reduction (a read-only pass) and the collapse
+    kernel with an empty mask (an in-place read + write pass with one multiply per amplitude);
+    plus a 2 GiB device-to-device copy.  Kernel times come from the profiler's trace."""
+    for _ in range(repeats):
+        t0 = time.perf_counter()
+        sim.collapse_wavefunction([], [])
+        sim.synchronize()
+        log(f"stream ceiling: norm pass + in-place pass in {1e3 * (time.perf_counter() - t0):.2f} ms (wall)")
+    for _ in range(repeats):
+        calibrate_copy()
+
+
 TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "k_tile_pass_pf_random32_c64_traffic.json")
 
 
@@ -422,6 +436,8 @@ def main():
     ap.add_argument("--batch", type=int, default=512, help="circuits in the batched-variational extra")
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--calibrate-copy", action="store_true", help="add a known-size D2D copy (counter scaling)")
+    ap.add_argument("--stream-ceiling", action="store_true",
+                    help="after the headline, run read-only and in-place streaming passes and 2 GiB copies (profile them)")
     ap.add_argument("--no-qft-adder", action="store_true", help="skip the 30-qubit QFT + rotations + AddQuantum extra")
     ap.add_argument("--no-engine-list", action="store_true", help="skip the default-engine-list extra")
     args = ap.parse_args()
@@ -444,6 +460,8 @@ def main():
     sim, wall, n_gates, stats = gpu_window_run(args, cmds, n, args.precision, "headline")
     if args.calibrate_copy:
         calibrate_copy()
+    if args.stream_ceiling:
+        stream_ceiling(sim)
     value = n_gates / wall
     ms_per_step = 1e3 * wall / args.steps
 

@@ -4,6 +4,8 @@
 // cheat() and get_amplitude().
 #include "engine.hpp"
 
+#include "mfma_layout.hpp"
+
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
@@ -64,6 +66,12 @@ Engine::Engine(uint32_t seed, int precision, int fusion_mode, int planner, unsig
   tile_tmax_ = 12u;
   tile_rb_ = 4u;
   tile_lmin_ = c64_ ? 5u : 4u;
+  if (const char *env = std::getenv("PROJECTQ_AMD_MFMA")) {
+    if (!std::strcmp(env, "0")) mfma_mode_ = 0;
+    else if (!std::strcmp(env, "all")) mfma_mode_ = 2;
+    else if (!std::strcmp(env, "1")) mfma_mode_ = 1;
+    else throw ValueError("PROJECTQ_AMD_MFMA must be 0, 1 or all");
+  }
   if (const char *env = std::getenv("PROJECTQ_AMD_TILE")) {
     unsigned t = 0, rb = 0;
     if (std::sscanf(env, "%u:%u", &t, &rb) != 2 || t < 6 || t > (c64_ ? 13u : 12u) || (rb != 4 && rb != 5))
@@ -620,6 +628,21 @@ void Engine::exec_tile_program(const TileProgram &prog, unsigned rb) {
   count_pass(2.0 * (double)amp_bytes() * (double)amps());
 }
 
+// Measured at 32 qubits (complex64, 68.7 GB per pass; `low` = targets at positions 0-3):
+//   k = 5: VALU 15.0-16.2 ms, MFMA 13.5-14.6 ms; with low = 2 16.8-17.4 against 17.5 ms; with
+//          low = 3 29.1 against 26.0 ms
+//   k = 4: VALU 13.3-14.0 ms, MFMA 14.1-15.0 ms (the VALU form already streams at the in-place
+//          rate); low = 3: VALU 25.4, MFMA 24.5 ms
+//   k = 3: VALU 12.9-13.5, MFMA 13.4-14.7 ms; low >= 2: VALU 17.5-25.3, MFMA 16.2-19.6 ms
+bool Engine::use_mfma(int k, const std::vector<uint32_t> &tpos, uint64_t n_groups) const {
+  if (!c64_ || k < 3 || k > 5 || n_groups < kMfmaMinGroups || mfma_mode_ == 0) return false;
+  if (mfma_mode_ == 2) return true;
+  int low = 0;
+  for (uint32_t p : tpos) low += p < 4;
+  if (k == 5) return low != 2;
+  return k == 4 ? low >= 3 : low >= 2;
+}
+
 void Engine::exec_block(const Block &b) {
   const int k = (int)b.qubits.size();
   if (k < 1 || k > 5) throw ValueError("Gates with more than 5 qubits are not supported!");
@@ -636,6 +659,18 @@ void Engine::exec_block(const Block &b) {
   if (ins.count > n_) return;
   const uint64_t n_groups = amps() >> ins.count;
   const size_t D = size_t(1) << k;
+  if (use_mfma(k, tpos, n_groups)) {
+    const std::vector<float> table = mfma::operand_table(k, b.m.data());
+    const float *d_table = (const float *)upload(table.data(), table.size() * sizeof(float));
+    PassFeatures f{};
+    if (recording_) f = block_features(b);
+    hipEvent_t e0, e1;
+    begin_pass(&e0, &e1);
+    dev::apply_block_mfma((float2 *)psi_, n_groups, ins, cmask, tpos.data(), k, d_table, stream_);
+    end_pass(e0, e1, recording_ ? &f : nullptr);
+    count_pass(2.0 * (double)amp_bytes() * (double)(n_groups << k));
+    return;
+  }
   dispatch([&](auto tag) {
     using R = decltype(tag);
     std::vector<amp_t<R>> m(D * D);

@@ -151,6 +151,12 @@ class Engine {
   void begin_pass(hipEvent_t *e0, hipEvent_t *e1);
   void end_pass(hipEvent_t e0, hipEvent_t e1, const PassFeatures *f);
   unsigned tile_lmin_ = 5;
+  // complex64 dense blocks on the matrix cores: 0 never (PROJECTQ_AMD_MFMA=0), 1 where measured
+  // faster than the VALU form (the default; use_mfma), 2 every block of 3-5 qubits
+  // (PROJECTQ_AMD_MFMA=all)
+  int mfma_mode_ = 1;
+  static constexpr uint64_t kMfmaMinGroups = 4096;
+  bool use_mfma(int k, const std::vector<uint32_t> &tpos, uint64_t n_groups) const;
   bool recording_ = false;
   struct PassRecord {
     PassFeatures f;

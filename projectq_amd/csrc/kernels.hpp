@@ -59,6 +59,11 @@ template <typename R>
 void apply_block(amp_t<R> *psi, uint64_t n_groups, const BitInsert &ins, uint64_t cmask,
                  const uint32_t *tpos, int k, const amp_t<R> *d_mat, hipStream_t s);
 
+// the same dense block for a complex64 state on the matrix cores (k = 3, 4, 5): d_table is the
+// per-lane A-operand table of mfma::operand_table (mfma_layout.hpp)
+void apply_block_mfma(float2 *psi, uint64_t n_groups, const BitInsert &ins, uint64_t cmask, const uint32_t *tpos,
+                      int k, const float *d_table, hipStream_t s);
+
 // partials[0..kReduceGrid) <- block sums of |psi_i|^2 over i = insert(g) | val, g < n_sub
 template <typename R>
 void norm_masked(const amp_t<R> *psi, uint64_t n_sub, const BitInsert &ins, uint64_t val,

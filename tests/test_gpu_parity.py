@@ -362,3 +362,51 @@ def test_c64_norm_guard_on_long_unitary_sequences(fusion):
             sim.apply_controlled_gate(G.H, [i % n], [])
         sim.run()
     assert abs(h.overlap(h).real - ref.overlap(ref).real) <= 1e-6 * ref.overlap(ref).real
+
+
+@pytest.mark.parametrize("k", [3, 4, 5])
+def test_dense_blocks_c64_matrix_cores(k):
+    """complex64 dense blocks of 3-5 qubits with 0-3 random controls on FP32 MFMA (states of 2^20
+    amplitudes: at least 4096 groups per block; PROJECTQ_AMD_MFMA=all puts every such block on the
+    matrix cores); every block against the oracle, and against the VALU form of the same pass
+    (PROJECTQ_AMD_MFMA=0)."""
+    import os
+
+    rng = np.random.default_rng(300 + k)
+    n = 20
+    cmds = circuits.allocate(n)
+    for _ in range(10):
+        picks = rng.choice(n, size=k + int(rng.integers(0, 4)), replace=False)
+        a = rng.normal(size=(2**k, 2**k)) + 1j * rng.normal(size=(2**k, 2**k))
+        u, _ = np.linalg.qr(a)
+        cmds.append(commands.gate(u, [int(q) for q in picks[:k]], [int(q) for q in picks[k:]]))
+        cmds.append(commands.gate(G.H, [int(rng.integers(0, n))]))
+    o = OracleSimulator(5)
+    commands.execute(o, cmds, False)
+    want = o.cheat_numpy()[1]
+    states = []
+    for mfma in ("all", "0"):
+        os.environ["PROJECTQ_AMD_MFMA"] = mfma
+        try:
+            h = Simulator(5, precision="c64")
+            commands.execute(h, cmds, False)
+        finally:
+            del os.environ["PROJECTQ_AMD_MFMA"]
+        got = h.cheat_numpy()[1]
+        assert infidelity(want, got) <= 1e-6
+        assert np.max(np.abs(got - want)) <= 2e-6
+        states.append(got)
+    assert np.max(np.abs(states[0] - states[1])) <= 2e-6
+
+
+def test_fused_blocks_c64_matrix_cores_random_circuit():
+    """gate_fusion=True in complex64 at 20 qubits: the reference rule's 4- and 5-qubit blocks run on
+    the matrix cores; against the oracle's unfused run."""
+    n = 20
+    cmds = circuits.random_layers(n, 6, 10, 91, measure=False)
+    o = OracleSimulator(2)
+    commands.execute(o, cmds, False)
+    h = Simulator(2, precision="c64")
+    commands.execute(h, cmds, True)
+    h.run()
+    assert infidelity(o.cheat_numpy()[1], h.cheat_numpy()[1]) <= 1e-6
