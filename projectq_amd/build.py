@@ -42,7 +42,7 @@ FLAGS = [
     "-Wno-unused-command-line-argument",
 ]
 
-SOURCES = ["kernels.hip", "mlp.hip", "compare.hip", "noalloc.hip", "mfma.hip", "engine.cpp", "planner.cpp", "batch.cpp", "capi.cpp"]
+SOURCES = ["kernels.hip", "mlp.hip", "compare.hip", "noalloc.hip", "mfma.hip", "batch_energy.hip", "engine.cpp", "planner.cpp", "batch.cpp", "capi.cpp"]
 
 
 def _compile(src):

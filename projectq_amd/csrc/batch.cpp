@@ -163,22 +163,49 @@ void Batch::build(const BatchShape &shape) {
     launch_blocks_.push_back(blocks);
     launch_tmax_.push_back(tmax);
   }
-  // ---- energies
+  // ---- energies: one job per circuit, chunks of up to 4096 amplitudes per workgroup
   std::vector<dev::EnergyJob> ejobs(circ_.size());
-  std::vector<uint32_t> eblock;
-  std::vector<dev::PauliTerm> terms;
+  std::vector<dev::BatchTerm> terms;
+  uint32_t slots = 0;
   for (size_t c = 0; c < circ_.size(); ++c) {
     dev::EnergyJob &e = ejobs[c];
     e.state_off = state_off[c];
     e.n_amps = uint64_t(1) << circ_[c].n;
     e.chunk = std::min<uint64_t>(e.n_amps, 4096);
-    e.first_block = (uint32_t)eblock.size();
+    e.first_block = slots;
     e.n_blocks = (uint32_t)((e.n_amps + e.chunk - 1) / e.chunk);
+    slots += e.n_blocks;
     e.term_off = (uint32_t)terms.size();
     e.n_terms = (uint32_t)circ_[c].terms.size();
-    terms.insert(terms.end(), circ_[c].terms.begin(), circ_[c].terms.end());
-    for (uint32_t b = 0; b < e.n_blocks; ++b) eblock.push_back((uint32_t)c);
+    for (const dev::PauliTerm &t : circ_[c].terms) {
+      // (-1)^{popcount((i ^ x) & z)} = (-1)^{popcount(i & z)} (-1)^{popcount(x & z)}
+      double s = (__builtin_popcountll(t.x & t.z) & 1) ? -1.0 : 1.0;
+      // pairs (i, i ^ x): half the amplitudes, twice the weight
+      const uint32_t h = t.x ? 1u << (63 - __builtin_clzll(t.x)) : 0u;
+      if (t.x) s *= 2.0;
+      terms.push_back(dev::BatchTerm{(uint32_t)t.x, (uint32_t)t.z, h, 0u, s * t.c_re, s * t.c_im});
+    }
   }
+  // Workgroup w runs on XCD w mod 8.  Every circuit goes whole to one XCD (largest first, to the
+  // least loaded), so its partner reads stay in that XCD's L2; the XCDs' workgroup lists are
+  // interleaved and padded with idle workgroups to equal length.
+  constexpr int kXcds = 8;
+  std::vector<size_t> order(circ_.size());
+  for (size_t c = 0; c < order.size(); ++c) order[c] = c;
+  std::stable_sort(order.begin(), order.end(), [&](size_t a, size_t b) { return circ_[a].n > circ_[b].n; });
+  std::vector<std::vector<dev::EnergyBlock>> per_xcd(kXcds);
+  std::vector<uint64_t> load(kXcds, 0);
+  for (size_t c : order) {
+    const int x = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+    load[x] += ejobs[c].n_amps * std::max<uint32_t>(ejobs[c].n_terms, 1);
+    for (uint32_t b = 0; b < ejobs[c].n_blocks; ++b) per_xcd[x].push_back(dev::EnergyBlock{(uint32_t)c, b});
+  }
+  size_t longest = 0;
+  for (auto &v : per_xcd) longest = std::max(longest, v.size());
+  std::vector<dev::EnergyBlock> eblock;
+  for (size_t k = 0; k < longest; ++k)
+    for (int x = 0; x < kXcds; ++x)
+      eblock.push_back(k < per_xcd[x].size() ? per_xcd[x][k] : dev::EnergyBlock{dev::kNoJob, 0});
   e_blocks_ = (uint32_t)eblock.size();
   // ---- device buffers
   const size_t ab = c64_ ? 8 : 16;
@@ -199,7 +226,7 @@ void Batch::build(const BatchShape &shape) {
   d_ejobs_ = to_device(ejobs);
   d_eblock_ = to_device(eblock);
   d_terms_ = to_device(terms);
-  QS_HIP(hipMalloc(&d_partials_, std::max<size_t>(e_blocks_, 1) * sizeof(double)));
+  QS_HIP(hipMalloc(&d_partials_, std::max<size_t>(slots, 1) * sizeof(double)));
   QS_HIP(hipMalloc(&d_energies_, circ_.size() * sizeof(double)));
   // ---- capture one run into a graph
   QS_HIP(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));

@@ -60,9 +60,9 @@ class Batch {
   std::vector<std::vector<dev::BatchJob>> launch_jobs_;
   std::vector<uint32_t> launch_blocks_, launch_tmax_;
   dev::EnergyJob *d_ejobs_ = nullptr;
-  uint32_t *d_eblock_ = nullptr;
+  dev::EnergyBlock *d_eblock_ = nullptr;
   uint32_t e_blocks_ = 0;
-  dev::PauliTerm *d_terms_ = nullptr;
+  dev::BatchTerm *d_terms_ = nullptr;
   double *d_partials_ = nullptr, *d_energies_ = nullptr, *h_energies_ = nullptr;
   hipGraph_t graph_ = nullptr;
   hipGraphExec_t exec_ = nullptr;

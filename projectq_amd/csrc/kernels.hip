@@ -806,73 +806,6 @@ __global__ void k_batch_ones(amp_t<R> *__restrict__ psi, const uint64_t *__restr
   if (c < n) psi[off[c]] = make_amp<R>(1.0, 0.0);
 }
 
-template <typename R>
-__global__ __launch_bounds__(kBlock) void k_batch_energy(const amp_t<R> *__restrict__ psi,
-                                                         const EnergyJob *__restrict__ jobs,
-                                                         const uint32_t *__restrict__ block_job,
-                                                         const PauliTerm *__restrict__ terms,
-                                                         double *partials) {
-  __shared__ double lds[kBlock / 64];
-  const EnergyJob *job = jobs + block_job[blockIdx.x];
-  const amp_t<R> *st = psi + job->state_off;
-  const PauliTerm *tm = terms + job->term_off;
-  const int nt = (int)job->n_terms;
-  const uint64_t lo = (uint64_t)(blockIdx.x - job->first_block) * job->chunk;
-  const uint64_t hi = min(job->n_amps, lo + job->chunk);
-  Kahan acc;
-  for (uint64_t i = lo + threadIdx.x; i < hi; i += kBlock) {
-    const amp_t<R> a = st[i];
-    const double ar = re(a), ai = im(a);
-    double ur = 0.0, ui = 0.0;
-    // four partner loads in flight before they are used: the loop is bound by load latency
-    int t = 0;
-    for (; t + 4 <= nt; t += 4) {
-      amp_t<R> b[4];
-      uint64_t j[4];
-#pragma unroll
-      for (int u = 0; u < 4; ++u) {
-        j[u] = i ^ tm[t + u].x;
-        b[u] = st[j[u]];
-      }
-#pragma unroll
-      for (int u = 0; u < 4; ++u) {
-        const PauliTerm pt = tm[t + u];
-        double br = re(b[u]), bi = im(b[u]);
-        if (__popcll(j[u] & pt.z) & 1) {
-          br = -br;
-          bi = -bi;
-        }
-        ur += br * pt.c_re - bi * pt.c_im;
-        ui += br * pt.c_im + bi * pt.c_re;
-      }
-    }
-    for (; t < nt; ++t) {
-      const PauliTerm pt = tm[t];
-      const uint64_t j = i ^ pt.x;
-      const amp_t<R> b = st[j];
-      double br = re(b), bi = im(b);
-      if (__popcll(j & pt.z) & 1) {
-        br = -br;
-        bi = -bi;
-      }
-      ur += br * pt.c_re - bi * pt.c_im;
-      ui += br * pt.c_im + bi * pt.c_re;
-    }
-    acc.add(ar * ur + ai * ui);
-  }
-  double t = block_sum(acc.value(), lds);
-  if (threadIdx.x == 0) partials[blockIdx.x] = t;
-}
-
-__global__ void k_batch_energy_finish(const EnergyJob *__restrict__ jobs, int n_jobs,
-                                      const double *__restrict__ partials, double *energies) {
-  const int j = blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= n_jobs) return;
-  double s = 0.0;
-  for (uint32_t b = 0; b < jobs[j].n_blocks; ++b) s += partials[jobs[j].first_block + b];
-  energies[j] = s;
-}
-
 // ------------------------------------------------------------------------------ reductions
 template <typename R>
 __global__ __launch_bounds__(kBlock) void k_norm_masked(const amp_t<R> *__restrict__ psi,
@@ -1445,20 +1378,6 @@ void batch_tile_pass(amp_t<R> *psi, const BatchJob *jobs, const uint32_t *block_
 }
 
 template <typename R>
-void batch_energy(const amp_t<R> *psi, const EnergyJob *jobs, const uint32_t *block_job, uint32_t n_blocks,
-                  const PauliTerm *terms, double *partials, hipStream_t s) {
-  hipLaunchKernelGGL(k_batch_energy<R>, dim3(n_blocks), dim3(kBlock), 0, s, psi, jobs, block_job, terms, partials);
-  QS_HIP(hipGetLastError());
-}
-
-void batch_energy_finish(const EnergyJob *jobs, int n_jobs, const double *partials, double *energies,
-                         hipStream_t s) {
-  hipLaunchKernelGGL(k_batch_energy_finish, dim3((n_jobs + 255) / 256), dim3(256), 0, s, jobs, n_jobs, partials,
-                     energies);
-  QS_HIP(hipGetLastError());
-}
-
-template <typename R>
 void norm_masked(const amp_t<R> *psi, uint64_t n_sub, const BitInsert &ins, uint64_t val,
                  double *partials, hipStream_t s) {
   hipLaunchKernelGGL(k_norm_masked<R>, dim3(kReduceGrid), dim3(kBlock), 0, s, psi, n_sub, ins, val,
@@ -1627,8 +1546,6 @@ template void convert_amps<float, double>(const amp_t<float> *, amp_t<double> *,
   template void batch_tile_pass<R>(amp_t<R> *, const BatchJob *, const uint32_t *, uint32_t,       \
                                    uint32_t, int, const TileStage *, const RegOp *, const TileOp *,\
                                    const amp_t<R> *, hipStream_t);                                \
-  template void batch_energy<R>(const amp_t<R> *, const EnergyJob *, const uint32_t *, uint32_t,   \
-                                const PauliTerm *, double *, hipStream_t);                        \
   template void tile_pass<R>(amp_t<R> *, uint64_t, const TilePass &, int, const TileStage *,      \
                              const RegOp *, const TileOp *, const amp_t<R> *, hipStream_t);       \
   template void apply_block<R>(amp_t<R> *, uint64_t, const BitInsert &, uint64_t,                 \

@@ -47,8 +47,8 @@ void batch_tile_pass(amp_t<R> *psi, const BatchJob *jobs, const uint32_t *block_
                      const amp_t<R> *mats, hipStream_t s);
 // partials[b] for every energy workgroup, then energies[j] = sum of job j's partials in order
 template <typename R>
-void batch_energy(const amp_t<R> *psi, const EnergyJob *jobs, const uint32_t *block_job, uint32_t n_blocks,
-                  const PauliTerm *terms, double *partials, hipStream_t s);
+void batch_energy(const amp_t<R> *psi, const EnergyJob *jobs, const EnergyBlock *blocks, uint32_t n_blocks,
+                  const BatchTerm *terms, double *partials, hipStream_t s);
 void batch_energy_finish(const EnergyJob *jobs, int n_jobs, const double *partials, double *energies,
                          hipStream_t s);
 

@@ -87,14 +87,31 @@ struct BatchJob {
   TilePass P;
 };
 
-// Expectation value of one circuit inside a batched launch (k_batch_energy): workgroups
-// first_block .. first_block + n_blocks - 1 each cover `chunk` amplitudes.
+// Expectation value of one circuit inside a batched launch (k_batch_energy): its amplitudes in
+// n_blocks chunks of `chunk`, whose partial sums go to partials[first_block + chunk index].
 struct EnergyJob {
   uint64_t state_off;
   uint64_t n_amps;
   uint64_t chunk;
   uint32_t first_block, n_blocks;
   uint32_t term_off, n_terms;
+};
+
+// One workgroup of the batched energy launch: chunk `chunk` of energy job `job`; kNoJob marks an
+// idle workgroup (the launch pads every XCD's share of workgroups to the same length).
+constexpr uint32_t kNoJob = 0xffffffffu;
+struct EnergyBlock {
+  uint32_t job, chunk;
+};
+
+// A Pauli string of a batched circuit (at most 30 qubits) with the constant part of its sign
+// folded into the coefficient: (P psi)_i = c (-1)^{popcount(i & z)} psi_{i ^ x}.  For x != 0 the
+// amplitudes i and i ^ x contribute complex-conjugate terms to <psi|P|psi> (P is Hermitian, the
+// coefficient real), so only the i with bit `h` (the highest bit of x) clear are summed and the
+// coefficient carries the factor 2; for x = 0, h = 0 and every i is summed.
+struct BatchTerm {
+  uint32_t x, z, h, pad;
+  double c_re, c_im;
 };
 
 // LDS slot of tile-local index x: an XOR swizzle that spreads the accesses of a register stage

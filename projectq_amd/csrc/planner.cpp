@@ -324,6 +324,146 @@ void emit_reg_stage(const std::vector<LocalGate> &run, uint32_t bits, unsigned t
   close_stage();
 }
 
+// Stages of one pass.  Within the tile, a non-diagonal gate writes its target bits; its tile-local
+// controls, and the targets of a diagonal gate, are only read (the same commutation rules as the
+// tile scheduler; controls outside the tile are read by every gate of the pass and never
+// conflict).  Repeatedly: the first unplaced gate is always ready; a gate of 3-5 targets becomes
+// an LDS stage; otherwise its targets are the forced register bits and the rest is chosen among
+// the bits the next unplaced register gates target: every completion is tried and the one under
+// which the stage absorbs the most gates wins, ties going to the fewest LDS bank conflicts (if
+// those gates target fewer bits than are free, the remainder is the conflict-optimal padding).
+// The stage takes -- sweeping in queue order until nothing more fits -- every ready gate whose
+// targets lie in its bits and which has at most one control among them.  Gates move only past
+// gates they commute with, so every bit sees its writes in the queued order.
+void partition_stages(const std::vector<LocalGate> &lgs, const std::vector<char> &diag, unsigned t, unsigned rb,
+                      bool c64, TileProgram &prog) {
+  const int n = (int)lgs.size();
+  std::vector<uint32_t> tm(n), wr(n), rd(n);
+  std::vector<char> lds_only(n);
+  for (int i = 0; i < n; ++i) {
+    for (uint32_t l = 0; l < lgs[i].k; ++l) tm[i] |= 1u << lgs[i].tb[l];
+    wr[i] = diag[i] ? 0u : tm[i];
+    rd[i] = lgs[i].lc | (diag[i] ? tm[i] : 0u);
+    lds_only[i] = lgs[i].k >= 3 || t < rb;
+  }
+  std::vector<std::vector<int>> preds(n);
+  {
+    std::vector<int> writer(32, -1);
+    std::vector<std::vector<int>> readers(32);
+    for (int i = 0; i < n; ++i) {
+      std::set<int> ps;
+      for (uint32_t b = 0; b < 32; ++b) {
+        if (!(((wr[i] | rd[i]) >> b) & 1)) continue;
+        if (writer[b] >= 0) ps.insert(writer[b]);
+        if ((wr[i] >> b) & 1) {
+          ps.insert(readers[b].begin(), readers[b].end());
+          readers[b].clear();
+          writer[b] = i;
+        } else {
+          readers[b].push_back(i);
+        }
+      }
+      preds[i].assign(ps.begin(), ps.end());
+    }
+  }
+  std::vector<char> done(n, 0);
+  auto ready = [&](int i) {
+    for (int p : preds[i])
+      if (!done[p]) return false;
+    return true;
+  };
+  int first = 0;
+  while (first < n) {
+    const int f = first;
+    if (lds_only[f]) {
+      emit_lds_op(lgs[f], t, prog);
+      done[f] = 1;
+    } else {
+      // the bits the next unplaced register gates target (lookahead window)
+      constexpr int kWindow = 48;
+      std::vector<int> window;
+      uint32_t cand = 0;
+      for (int i = f; i < n && (int)window.size() < kWindow; ++i) {
+        if (done[i] || lds_only[i]) continue;
+        window.push_back(i);
+        cand |= tm[i];
+      }
+      cand &= ~tm[f];
+      const uint32_t lane_bits = c64 ? 5u : 4u;
+      const uint32_t free_slots = rb - (uint32_t)popcount32(tm[f]);
+      uint32_t bits;
+      if ((uint32_t)popcount32(cand) <= free_slots) {
+        bits = pad_stage_bits(tm[f] | cand, t, rb, c64);
+      } else {
+        // gates a stage with these bits would take, simulated over the window
+        std::vector<char> sim(n);
+        auto absorbed = [&](uint32_t b) {
+          for (int i : window) sim[i] = done[i];
+          int got = 0;
+          for (bool progress = true; progress;) {
+            progress = false;
+            for (int i : window) {
+              if (sim[i] || (tm[i] & ~b) || popcount32(lgs[i].lc & b) > 1) continue;
+              bool ok = true;
+              for (int p : preds[i])
+                if (!(p >= f ? sim[p] : done[p])) {
+                  ok = false;
+                  break;
+                }
+              if (!ok) continue;
+              sim[i] = 1;
+              ++got;
+              progress = true;
+            }
+          }
+          return got;
+        };
+        std::vector<uint32_t> cbits;
+        for (uint32_t b = 0; b < t; ++b)
+          if ((cand >> b) & 1) cbits.push_back(b);
+        std::vector<uint32_t> idx(free_slots);
+        for (uint32_t i = 0; i < free_slots; ++i) idx[i] = i;
+        int best_got = -1;
+        uint32_t best_conf = ~0u;
+        bits = 0;
+        while (true) {
+          uint32_t b = tm[f];
+          for (uint32_t i : idx) b |= 1u << cbits[i];
+          const int got = absorbed(b);
+          const uint32_t conf = dev::stage_conflict_degree(b, t, c64, lane_bits);
+          if (got > best_got || (got == best_got && conf < best_conf)) {
+            best_got = got;
+            best_conf = conf;
+            bits = b;
+          }
+          int i = (int)free_slots - 1;
+          while (i >= 0 && idx[i] == cbits.size() - free_slots + (uint32_t)i) --i;
+          if (i < 0) break;
+          ++idx[i];
+          for (uint32_t j = (uint32_t)i + 1; j < free_slots; ++j) idx[j] = idx[j - 1] + 1;
+        }
+      }
+      std::vector<LocalGate> run;
+      for (bool progress = true; progress;) {
+        progress = false;
+        for (int i = f; i < n; ++i) {
+          if (done[i] || lds_only[i] || (tm[i] & ~bits) || popcount32(lgs[i].lc & bits) > 1 || !ready(i)) continue;
+          done[i] = 1;
+          run.push_back(lgs[i]);
+          progress = true;
+        }
+      }
+      if (run.empty()) {  // the first gate has two controls among the padded bits: LDS op
+        emit_lds_op(lgs[f], t, prog);
+        done[f] = 1;
+      } else {
+        emit_reg_stage(run, bits, t, rb, c64, prog);
+      }
+    }
+    while (first < n && done[first]) ++first;
+  }
+}
+
 }  // namespace
 
 void build_tile_pass(const std::vector<PGate> &gates, const PPass &ps, unsigned t, unsigned rb, bool c64,
@@ -353,13 +493,8 @@ void build_tile_pass(const std::vector<PGate> &gates, const PPass &ps, unsigned 
   P.tiles.count = (uint32_t)tilepos.size();
   for (size_t i = 0; i < tilepos.size(); ++i) P.tiles.pos[i] = (uint8_t)tilepos[i];
 
-  std::vector<LocalGate> run;
-  uint32_t run_bits = 0;
-  auto flush = [&]() {
-    if (!run.empty()) emit_reg_stage(run, run_bits, t, rb, c64, prog);
-    run.clear();
-    run_bits = 0;
-  };
+  std::vector<LocalGate> lgs;
+  std::vector<char> diag;
   for (int gi : ps.gates) {
     const PGate &g = gates[gi];
     LocalGate lg{};
@@ -378,16 +513,10 @@ void build_tile_pass(const std::vector<PGate> &gates, const PPass &ps, unsigned 
       else lg.gc |= uint64_t(1) << p;
     }
     lg.m = &g.m;
-    if (lg.k >= 3 || t < rb) {
-      flush();
-      emit_lds_op(lg, t, prog);
-      continue;
-    }
-    if (popcount32(run_bits | need) > rb) flush();
-    run_bits |= need;
-    run.push_back(lg);
+    lgs.push_back(lg);
+    diag.push_back(diagonal_gate(g) ? 1 : 0);
   }
-  flush();
+  partition_stages(lgs, diag, t, rb, c64, prog);
   P.n_ops = (uint32_t)(prog.rops.size() + prog.tops.size());
   P.n_stages = (uint32_t)prog.stages.size();
 }
