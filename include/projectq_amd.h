@@ -208,6 +208,12 @@ int qs_compare_states(qs_simulator *a, qs_simulator *b, double *max_abs_diff, ui
  * Tile shape of QS_PLANNER_TILE: tile bits (6..13; 12 for complex128), register bits (4 or 5) and
  * low contiguous bits every tile keeps. */
 int qs_set_tile_params(qs_simulator *sim, uint32_t tile_bits, uint32_t reg_bits, uint32_t low_bits);
+/* Rule for the free tile slots of QS_PLANNER_TILE (the groupings its passes get): */
+#define QS_SLOT_LOOKAHEAD 0 /* highest decayed lookahead score (default) */
+#define QS_SLOT_EARLIEST 1  /* positions the earliest unscheduled gates need, in order of need */
+int qs_set_slot_rule(qs_simulator *sim, int rule);
+/* The slot rule of the last window's tile plan (the learned planner tries both). */
+int qs_last_plan_rule(qs_simulator *sim, int *rule);
 /* Record every window pass: QS_PASS_FEATURES features and its GPU time. */
 #define QS_PASS_FEATURES 24
 int qs_set_recording(qs_simulator *sim, int enabled);

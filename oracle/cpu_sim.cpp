@@ -935,6 +935,11 @@ API int qs_compare_states(qs_simulator *a, qs_simulator *b, double *max_abs_diff
 // Planner controls and the MLP are GPU-side features with no reference counterpart; the oracle
 // only answers for the simulator surface.
 API int qs_set_tile_params(qs_simulator *, uint32_t, uint32_t, uint32_t) { return QS_OK; }
+API int qs_set_slot_rule(qs_simulator *, int) { return QS_OK; }
+API int qs_last_plan_rule(qs_simulator *, int *rule) {
+  *rule = 0;
+  return QS_OK;
+}
 API int qs_set_recording(qs_simulator *, int) { return QS_OK; }
 API int qs_take_records(qs_simulator *, float *, float *, size_t, size_t *rows) {
   *rows = 0;

@@ -111,6 +111,8 @@ _SIGNATURES = {
     ),
     "qs_compare_states": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, _f64p, ctypes.POINTER(ctypes.c_uint64)]),
     "qs_set_tile_params": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32]),
+    "qs_set_slot_rule": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
+    "qs_last_plan_rule": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int)]),
     "qs_set_recording": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     "qs_take_records": (
         ctypes.c_int,
@@ -586,6 +588,12 @@ class CAbiSimulator:
     def set_tile_params(self, tile_bits, reg_bits, low_bits):
         self._check(self._lib.qs_set_tile_params(self._handle, int(tile_bits), int(reg_bits), int(low_bits)))
 
+    SLOT_RULES = ("lookahead", "earliest")
+
+    def set_slot_rule(self, rule):
+        """Slot rule of the heuristic tile planner: "lookahead" (default) or "earliest"."""
+        self._check(self._lib.qs_set_slot_rule(self._handle, self.SLOT_RULES.index(rule)))
+
     def set_recording(self, enabled=True):
         self._check(self._lib.qs_set_recording(self._handle, 1 if enabled else 0))
 
@@ -624,8 +632,11 @@ class CAbiSimulator:
                 ctypes.byref(pred),
             )
         )
+        rule = ctypes.c_int()
+        self._check(self._lib.qs_last_plan_rule(self._handle, ctypes.byref(rule)))
         return {
             "kind": {0: "greedy_blocks", 1: "tile"}.get(kind.value, "none"),
+            "rule": self.SLOT_RULES[rule.value] if kind.value == 1 else None,
             "tile_bits": t.value,
             "reg_bits": rb.value,
             "low_bits": low.value,

@@ -353,6 +353,12 @@ API void qs_batch_destroy(qs_batch *b) {
 API int qs_set_tile_params(qs_simulator *s, uint32_t t, uint32_t rb, uint32_t low) {
   return guarded(s, [&] { s->engine.set_tile_params(t, rb, low); });
 }
+API int qs_set_slot_rule(qs_simulator *s, int rule) {
+  return guarded(s, [&] { s->engine.set_slot_rule(rule); });
+}
+API int qs_last_plan_rule(qs_simulator *s, int *rule) {
+  return guarded(s, [&] { *rule = s->engine.last_plan().rule; });
+}
 API int qs_set_recording(qs_simulator *s, int on) {
   return guarded(s, [&] { s->engine.set_recording(on != 0); });
 }

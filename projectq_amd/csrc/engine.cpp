@@ -7,7 +7,9 @@
 #include "mfma_layout.hpp"
 
 #include <algorithm>
+#include <array>
 #include <cmath>
+#include <future>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -473,10 +475,10 @@ unsigned Engine::tile_bits(unsigned tmax) const {
 }
 
 std::vector<TileProgram> Engine::plan_tiles(const std::vector<PGate> &pg, unsigned tmax, unsigned rb,
-                                            unsigned lmin, unsigned *t_used) const {
+                                            unsigned lmin, int rule, unsigned *t_used) const {
   const unsigned t = tile_bits(tmax);
   *t_used = t;
-  std::vector<PPass> passes = schedule_tiles(pg, TileShape{n_, t, lmin});
+  std::vector<PPass> passes = schedule_tiles(pg, TileShape{n_, t, lmin}, slot_chooser(rule));
   std::vector<TileProgram> progs(passes.size());
   for (size_t i = 0; i < passes.size(); ++i) build_tile_pass(pg, passes[i], t, rb, c64_, progs[i]);
   return progs;
@@ -523,29 +525,42 @@ void Engine::plan_window(const std::vector<Gate> &gates, std::vector<Block> *blo
   const std::vector<PGate> pg = resolve(gates);
   if (planner_kind_ == 2) {
     unsigned t;
-    *progs_out = plan_tiles(pg, tile_tmax_, tile_rb_, tile_lmin_, &t);
+    *progs_out = plan_tiles(pg, tile_tmax_, tile_rb_, tile_lmin_, slot_rule_, &t);
     *rb_out = tile_rb_;
-    *info = PlanInfo{1, tile_tmax_, tile_rb_, tile_lmin_, (unsigned)progs_out->size(), 1, 0.0};
+    *info = PlanInfo{1, tile_tmax_, tile_rb_, tile_lmin_, (unsigned)progs_out->size(), 1, slot_rule_, 0.0};
     return;
   }
-  // learned planner: predict the time of every pass of every candidate plan, run the cheapest
+  // learned planner: candidate plans of every tile shape under every slot rule (the groupings
+  // differ), built concurrently on host threads, and the greedy max-5 blocks; the cost model
+  // predicts the time of every pass of every candidate and the cheapest plan runs
   struct Cand {
     unsigned tmax, rb, lmin;
+    int rule;
   };
   std::vector<Cand> cands;
-  if (c64_) cands = {{12, 4, 5}, {12, 4, 6}, {12, 4, 4}, {11, 4, 5}, {10, 4, 4}, {12, 5, 5}, {13, 5, 5}};
-  else cands = {{12, 4, 4}, {12, 4, 5}, {11, 4, 4}, {10, 4, 4}, {12, 5, 4}};
-  std::vector<std::vector<TileProgram>> plans;
+  const std::vector<std::array<unsigned, 3>> shapes =
+      c64_ ? std::vector<std::array<unsigned, 3>>{{12, 4, 5}, {12, 4, 6}, {12, 4, 4}, {11, 4, 5}, {10, 4, 4}, {12, 5, 5}, {13, 5, 5}}
+           : std::vector<std::array<unsigned, 3>>{{12, 4, 4}, {12, 4, 5}, {11, 4, 4}, {10, 4, 4}, {12, 5, 4}};
+  for (int rule = 0; rule < kSlotRules; ++rule)
+    for (const auto &s : shapes) cands.push_back(Cand{s[0], s[1], s[2], rule});
+  std::vector<std::vector<TileProgram>> plans(cands.size());
+  {
+    std::vector<std::future<void>> work;
+    for (size_t c = 0; c < cands.size(); ++c)
+      work.push_back(std::async(std::launch::async, [&, c] {
+        unsigned t;
+        plans[c] = plan_tiles(pg, cands[c].tmax, cands[c].rb, cands[c].lmin, cands[c].rule, &t);
+      }));
+    for (auto &w : work) w.get();
+  }
   std::vector<float> X;
   std::vector<size_t> rows_of;
-  for (const Cand &c : cands) {
-    unsigned t;
-    plans.push_back(plan_tiles(pg, c.tmax, c.rb, c.lmin, &t));
-    for (const TileProgram &p : plans.back()) {
-      PassFeatures f = tile_pass_features(p, n_, c64_, c.rb);
+  for (size_t c = 0; c < cands.size(); ++c) {
+    for (const TileProgram &p : plans[c]) {
+      PassFeatures f = tile_pass_features(p, n_, c64_, cands[c].rb);
       X.insert(X.end(), f.f, f.f + kPassFeatures);
     }
-    rows_of.push_back(plans.back().size());
+    rows_of.push_back(plans[c].size());
   }
   std::vector<Block> blocks = greedy_blocks(gates);
   for (const Block &b : blocks) {
@@ -577,6 +592,7 @@ void Engine::plan_window(const std::vector<Gate> &gates, std::vector<Block> *blo
   info->tmax = c.tmax;
   info->rb = c.rb;
   info->lmin = c.lmin;
+  info->rule = c.rule;
   info->passes = (unsigned)plans[best].size();
   *rb_out = c.rb;
   *progs_out = std::move(plans[best]);
@@ -698,6 +714,12 @@ void Engine::set_tile_params(unsigned tmax, unsigned rb, unsigned lmin) {
   tile_tmax_ = tmax;
   tile_rb_ = rb;
   tile_lmin_ = lmin;
+}
+
+void Engine::set_slot_rule(int rule) {
+  if (rule < 0 || rule >= kSlotRules) throw ValueError("unknown tile slot rule");
+  run();
+  slot_rule_ = rule;
 }
 
 void Engine::set_recording(bool on) {

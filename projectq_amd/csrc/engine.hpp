@@ -73,12 +73,14 @@ class Engine {
 
   // planner controls (C-ABI extensions)
   void set_tile_params(unsigned tmax, unsigned rb, unsigned lmin);
+  void set_slot_rule(int rule);
   void set_recording(bool on);
   size_t take_records(float *features, float *ms, size_t cap);
   void set_model(const std::vector<float> &w);
   struct PlanInfo {
     int kind = -1;  // 0 greedy blocks, 1 tile passes
     unsigned tmax = 0, rb = 0, lmin = 0, passes = 0, candidates = 0;
+    int rule = 0;   // tile slot rule (SlotRule)
     double predicted_ms = 0.0;
   };
   const PlanInfo &last_plan() const { return last_plan_; }
@@ -145,7 +147,8 @@ class Engine {
   std::vector<Block> greedy_blocks(const std::vector<Gate> &gates) const;
   std::vector<PGate> resolve(const std::vector<Gate> &gates) const;
   std::vector<TileProgram> plan_tiles(const std::vector<PGate> &pg, unsigned tmax, unsigned rb, unsigned lmin,
-                                      unsigned *t_used) const;
+                                      int rule, unsigned *t_used) const;
+  int slot_rule_ = kSlotLookahead;  // rule of QS_PLANNER_TILE
   void exec_tile_program(const TileProgram &prog, unsigned rb);
   PassFeatures block_features(const Block &b) const;
   void begin_pass(hipEvent_t *e0, hipEvent_t *e1);

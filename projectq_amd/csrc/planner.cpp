@@ -50,6 +50,25 @@ std::vector<uint32_t> choose_by_score(const std::vector<SlotCandidate> &cands, u
   return out;
 }
 
+std::vector<uint32_t> choose_earliest(const std::vector<SlotCandidate> &cands, unsigned slots) {
+  std::vector<SlotCandidate> c = cands;
+  std::stable_sort(c.begin(), c.end(), [](const SlotCandidate &a, const SlotCandidate &b) {
+    return a.first_rank != b.first_rank ? a.first_rank < b.first_rank : a.lookahead_score > b.lookahead_score;
+  });
+  std::vector<uint32_t> out;
+  for (auto &x : c) {
+    if (out.size() >= slots) break;
+    out.push_back(x.pos);
+  }
+  return out;
+}
+
+SlotChooser slot_chooser(int rule) {
+  if (rule == kSlotLookahead) return nullptr;
+  if (rule == kSlotEarliest) return choose_earliest;
+  throw std::invalid_argument("unknown tile slot rule");
+}
+
 // A gate whose matrix is diagonal commutes with controls on its targets and with other diagonal
 // gates there, so for dependencies its targets are read, like controls.
 static bool diagonal_gate(const PGate &g) {
@@ -62,7 +81,7 @@ static bool diagonal_gate(const PGate &g) {
 }
 
 std::vector<PPass> schedule_tiles(const std::vector<PGate> &gates, const TileShape &shape,
-                                  const SlotChooser &choose, unsigned lookahead) {
+                                  const SlotChooser &choose, unsigned lookahead, double decay) {
   const int G = (int)gates.size();
   const unsigned n = shape.n, t = std::min(shape.t, shape.n);
   const unsigned L0 = std::min(shape.Lmin, t);
@@ -110,7 +129,7 @@ std::vector<PPass> schedule_tiles(const std::vector<PGate> &gates, const TileSha
         if (done[i]) continue;
         for (uint32_t p : gates[i].tpos) {
           if (p < L0) continue;
-          score[p] += 1.0 / (1.0 + rank / 16.0);
+          score[p] += 1.0 / (1.0 + rank / decay);
           cnt[p] += 1.0;
           if (firstr[p] < 0) firstr[p] = rank;
         }

@@ -76,7 +76,7 @@ def main():
     ]
     for n, g, name, st, wall, inf, plan in rows:
         plan_s = "" if plan is None else (
-            f"{plan['kind']} t={plan['tile_bits']} rb={plan['reg_bits']} L={plan['low_bits']}" if plan["kind"] == "tile" else plan["kind"]
+            f"{plan['kind']} t={plan['tile_bits']} rb={plan['reg_bits']} L={plan['low_bits']} {plan['rule']}" if plan["kind"] == "tile" else plan["kind"]
         )
         inf_s = "" if inf is None else f"{inf:.2e}"
         lines.append(

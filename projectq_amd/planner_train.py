@@ -4,7 +4,8 @@
 
 1. Generate seeded random circuits (24..30 qubits, mixed 1/2/3-qubit gates or random layers with
    CNOTs), run each once in window mode with a random planner configuration (tile passes of a
-   random shape, or greedy max-5 blocks), recording every pass: its features and its GPU time.
+   random shape under a random slot rule, or greedy max-5 blocks), recording every pass: its
+   features and its GPU time.
 2. Train the 24 -> 64 -> 64 -> 1 MLP on log(time per 2^20 amplitudes) with the HIP training
    kernels (qs_mlp_train), holding out 10% of the circuits for validation.
 3. Save the weights with the training statistics.
@@ -53,6 +54,7 @@ def generate(n_circuits, qmin, qmax, seed):
                         planner="greedy" if greedy else "tile", capacity_qubits=n)
         if not greedy:
             sim.set_tile_params(*TILE_SHAPES[prec][int(rng.integers(len(TILE_SHAPES[prec])))])
+            sim.set_slot_rule(Simulator.SLOT_RULES[int(rng.integers(len(Simulator.SLOT_RULES)))])
         for q in range(n):
             sim.allocate_qubit(q)
         sim.run()

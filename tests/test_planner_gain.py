@@ -41,8 +41,11 @@ def test_model_is_shipped():
     assert w is not None and w.shape == (5825,) and np.all(np.isfinite(w))
 
 
-@pytest.mark.parametrize("n,depth", [(22, 40), (25, 30)])
-def test_learned_planner_gain_and_parity(n, depth):
+@pytest.mark.parametrize("n,depth,over_tile", [(22, 40, None), (25, 30, None), (28, 100, 1.15)])
+def test_learned_planner_gain_and_parity(n, depth, over_tile):
+    """over_tile: the least gain over the heuristic tile planner the learned planner must show
+    (its candidates include groupings under the earliest-need slot rule, which the heuristic does
+    not use; on depth-100 mixed circuits it measured 1.21-1.31x at 28-32 qubits)."""
     gates = [c for c in circuits.mixed_circuit(n, depth, 900 + n) if c[0] == "gate"]
     ref, _ = execute(n, gates, precision="c128", fusion="reference")
     results = {}
@@ -59,6 +62,8 @@ def test_learned_planner_gain_and_parity(n, depth):
     assert results["learned"][3]["candidates"] >= 2
     assert learned < greedy
     assert learned <= 1.25 * tile
+    if over_tile:
+        assert tile / learned >= over_tile, tile / learned
 
 
 def test_learned_planner_c128_matches_oracle():
