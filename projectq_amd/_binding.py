@@ -55,7 +55,7 @@ _SIGNATURES = {
     "qs_measure_qubits": (ctypes.c_int, [ctypes.c_void_p, _u32p, ctypes.c_size_t, _u8p]),
     "qs_apply_controlled_gate": (
         ctypes.c_int,
-        [ctypes.c_void_p, _f64p, _u32p, ctypes.c_size_t, _u32p, ctypes.c_size_t],
+        [ctypes.c_void_p, ctypes.c_void_p, _u32p, ctypes.c_size_t, _u32p, ctypes.c_size_t],
     ),
     "qs_emulate_math": (
         ctypes.c_int,
@@ -219,6 +219,17 @@ def _u32(seq):
     return arr.astype(np.uint32)
 
 
+def _u32_array(seq):
+    """Qubit ids as a ctypes uint32 array; ids outside [0, 2^32) raise TypeError, as _u32 does."""
+    if not isinstance(seq, (list, tuple)):
+        arr = _u32(seq)
+        return (ctypes.c_uint32 * arr.size)(*arr.tolist())
+    for v in seq:
+        if not 0 <= v <= 0xFFFFFFFF:
+            raise TypeError("qubit ids must be unsigned 32-bit integers")
+    return (ctypes.c_uint32 * len(seq))(*seq)
+
+
 def _bits(seq):
     """Booleans as uint8; '0'/'1' characters count as their digit (_simulator.py:221, :247)."""
     return np.ascontiguousarray(
@@ -353,22 +364,17 @@ class CAbiSimulator:
         return [bool(v) for v in out]
 
     def apply_controlled_gate(self, m, ids, ctrl):
-        ids = _u32(ids)
-        ctrl = _u32(ctrl)
-        mat = np.ascontiguousarray(np.asarray(m, dtype=np.complex128))
+        # the per-gate path of every circuit: short id lists go straight into ctypes arrays and a
+        # complex128 C-contiguous matrix is passed by address, without intermediate numpy copies
+        ids = _u32_array(ids)
+        ctrl = _u32_array(ctrl)
+        mat = m
+        if not (type(mat) is np.ndarray and mat.dtype == np.complex128 and mat.flags.c_contiguous):
+            mat = np.ascontiguousarray(np.asarray(m, dtype=np.complex128))
         dim = 1 << len(ids)
         if mat.shape != (dim, dim):
             raise ValueError(f"apply_controlled_gate: a gate on {len(ids)} qubits needs a {dim}x{dim} matrix, got {mat.shape}")
-        self._check(
-            self._lib.qs_apply_controlled_gate(
-                self._handle,
-                _ptr(mat.view(np.float64), ctypes.c_double),
-                _ptr(ids, ctypes.c_uint32),
-                len(ids),
-                _ptr(ctrl, ctypes.c_uint32),
-                len(ctrl),
-            )
-        )
+        self._check(self._lib.qs_apply_controlled_gate(self._handle, mat.ctypes.data, ids, len(ids), ctrl, len(ctrl)))
 
     def _math_op(self, opcode, params, quregs, ctrl):
         flat, sizes = _regs(quregs)

@@ -315,6 +315,9 @@ API int qs_batch_create(int precision, size_t n_circuits, const uint32_t *n_qubi
         qs::dev::PauliTerm pt{};
         qs::cplx w;
         qs::encode_pauli(ops, &pt.x, &pt.z, &w);
+        // the energy kernel pairs i with i ^ x, which needs a Hermitian string; an anti-Hermitian
+        // one (a qubit named twice) has Re<psi|P|psi> = 0 and is left out
+        if (!qs::dev::pauli_hermitian(pt.x, pt.z, w)) continue;
         const qs::cplx c = coeffs[t] * w;
         pt.c_re = c.real();
         pt.c_im = c.imag();

@@ -955,6 +955,10 @@ double Engine::get_expectation_value(const std::vector<PauliString> &terms,
   const bool cached = amps() * amp_bytes() <= (uint64_t(64) << 20);
   if (cached && !pt.empty()) {
     for (size_t t = 0; t < pt.size(); ++t) {
+      if (!dev::pauli_hermitian(pt[t].x, pt[t].z, omega[t])) {
+        pt[t] = dev::PauliTerm{0, 0, 0.0, 0.0};  // Re<psi|P|psi> = 0 for an anti-Hermitian string
+        continue;
+      }
       const cplx c = coeffs[t] * omega[t];
       pt[t].c_re = c.real();
       pt[t].c_im = c.imag();
@@ -968,6 +972,7 @@ double Engine::get_expectation_value(const std::vector<PauliString> &terms,
   }
   double total = 0.0;
   for (size_t t = 0; t < terms.size(); ++t) {
+    if (!dev::pauli_hermitian(pt[t].x, pt[t].z, omega[t])) continue;  // Re<psi|P|psi> = 0
     double delta;
     if (pt[t].x == 0 && pt[t].z == 0) {
       delta = omega[t].real() * masked_norm(0, 0);

@@ -188,15 +188,15 @@ __device__ __forceinline__ void lds_store_amp(uint32_t lds_byte_addr, const amp_
   lds_store<re2_t<R>>(lds_byte_addr, v);
 }
 
-// One TileOp of any width straight on the LDS tile.
-template <typename R, int K>
+// One TileOp of any width straight on the LDS tile (NT threads per workgroup).
+template <typename R, int K, int NT = kBlock>
 __device__ __forceinline__ void tile_apply(amp_t<R> *__restrict__ tile, uint32_t t,
                                            const TileOp *__restrict__ op,
                                            const amp_t<R> *__restrict__ m) {
   constexpr int D = 1 << K;
   const uint32_t nfixed = op->nfixed, lcmask = op->lcmask;
   const uint32_t ngroups = 1u << (t - nfixed);
-  for (uint32_t g = threadIdx.x; g < ngroups; g += kBlock) {
+  for (uint32_t g = threadIdx.x; g < ngroups; g += NT) {
     uint32_t lb = g;
     for (uint32_t i = 0; i < nfixed; ++i) {
       const uint32_t p = op->fixed[i];
@@ -471,7 +471,7 @@ __device__ __forceinline__ void reg_apply(const RegOp *__restrict__ op, uint64_t
 // TURN ops per turn of the op loop.  With 2 the register allocator moves the amplitudes between
 // register sets once per turn rather than once per op (worth about 2% on the pass kernel), at
 // the price of a second copy of the op dispatch; kernels off the hot path use 1.
-template <typename R, int RB, int TURN>
+template <typename R, int RB, int TURN, int NT = kBlock>
 __device__ __forceinline__ void reg_stage(amp_t<R> *__restrict__ tile, uint32_t t, uint64_t base,
                                           const TileStage *__restrict__ st,
                                           const RegOp *__restrict__ rops,
@@ -482,7 +482,7 @@ __device__ __forceinline__ void reg_stage(amp_t<R> *__restrict__ tile, uint32_t 
   for (int i = 0; i < NR; ++i) soff[i] = st->soff[i];
   const uint32_t ngroups = 1u << (t - RB);
   const uint32_t ob = st->begin, oe = st->end;
-  for (uint32_t g = threadIdx.x; g < ngroups; g += kBlock) {
+  for (uint32_t g = threadIdx.x; g < ngroups; g += NT) {
     uint32_t lb = g;
 #pragma unroll
     for (int b = 0; b < RB; ++b) {
@@ -597,25 +597,25 @@ __device__ __forceinline__ uint64_t tile_base(uint64_t tix, const TilePass &P) {
 constexpr int kStreamChunk = 8;
 
 // The stages of a pass on one tile resident in LDS, a workgroup barrier after each.
-template <typename R, int RB, int TURN>
+template <typename R, int RB, int TURN, int NT = kBlock>
 __device__ __forceinline__ void run_stages(amp_t<R> *__restrict__ tile, uint32_t t, uint64_t base, uint32_t n_stages,
                                            const TileStage *__restrict__ stages, const RegOp *__restrict__ rops,
                                            const TileOp *__restrict__ tops, const amp_t<R> *__restrict__ mats) {
   for (uint32_t s = 0; s < n_stages; ++s) {
     const TileStage *st = stages + s;
     if (st->kind == kStageRegisters) {
-      reg_stage<R, RB, TURN>(tile, t, base, st, rops, mats);
+      reg_stage<R, RB, TURN, NT>(tile, t, base, st, rops, mats);
     } else {
       const TileOp *op = tops + st->begin;
       const uint64_t gc = op->gcmask;
       if ((base & gc) == gc) {
         const amp_t<R> *m = mats + op->mat;
         switch (op->k) {
-          case 1: tile_apply<R, 1>(tile, t, op, m); break;
-          case 2: tile_apply<R, 2>(tile, t, op, m); break;
-          case 3: tile_apply<R, 3>(tile, t, op, m); break;
-          case 4: tile_apply<R, 4>(tile, t, op, m); break;
-          default: tile_apply<R, 5>(tile, t, op, m); break;
+          case 1: tile_apply<R, 1, NT>(tile, t, op, m); break;
+          case 2: tile_apply<R, 2, NT>(tile, t, op, m); break;
+          case 3: tile_apply<R, 3, NT>(tile, t, op, m); break;
+          case 4: tile_apply<R, 4, NT>(tile, t, op, m); break;
+          default: tile_apply<R, 5, NT>(tile, t, op, m); break;
         }
       }
     }
@@ -672,8 +672,9 @@ __device__ __forceinline__ void tile_body(amp_t<R> *__restrict__ psi, uint64_t t
   __syncthreads();
 }
 
+template <int NT = kBlock>
 __device__ __forceinline__ void fill_segments(const TilePass &P, uint64_t *seg) {
-  for (uint32_t sgi = threadIdx.x; sgi < (1u << P.h); sgi += kBlock) {
+  for (uint32_t sgi = threadIdx.x; sgi < (1u << P.h); sgi += NT) {
     uint64_t o = 0;
     for (uint32_t b = 0; b < P.h; ++b)
       if ((sgi >> b) & 1) o |= uint64_t(1) << P.hpos[b];
@@ -701,22 +702,24 @@ __global__ __launch_bounds__(kBlock) void k_tile_pass(amp_t<R> *__restrict__ psi
 // Software-pipelined tile pass: while the stages of tile i run out of LDS, the loads of the
 // workgroup's next tile are already in flight into registers, so HBM traffic overlaps the in-tile
 // arithmetic.  Used when a thread's share of a tile is exactly kStreamChunk 16-byte vectors
-// (2^t = kStreamChunk * kBlock * VN), so all streaming is unguarded straight-line code.
-template <typename R, int RB>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) void k_tile_pass_pf(amp_t<R> *__restrict__ psi, uint64_t n_tiles,
+// (2^t = kStreamChunk * NT * VN), so all streaming is unguarded straight-line code.  NT = 256
+// serves 2^12-amplitude complex64 tiles (32 KiB of LDS, four workgroups per CU), NT = 512 tiles of
+// 2^13 (64 KiB, two workgroups per CU): the same per-thread work and the same waves per CU.
+template <typename R, int RB, int NT>
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_tile_pass_pf(amp_t<R> *__restrict__ psi, uint64_t n_tiles,
                                                          TilePass P, const TileStage *__restrict__ stages,
                                                          const RegOp *__restrict__ rops,
                                                          const TileOp *__restrict__ tops,
                                                          const amp_t<R> *__restrict__ mats) {
   using V = typename Vec16<R>::type;
   constexpr int VN = Vec16<R>::n;
-  constexpr uint32_t kStep = kBlock * VN;
+  constexpr uint32_t kStep = NT * VN;
   extern __shared__ __align__(16) unsigned char smem[];
   // the tile starts at LDS address 0 (no static LDS), so tile offsets need no base add;
   // the 2^h segment offsets follow the tile
   amp_t<R> *tile = reinterpret_cast<amp_t<R> *>(smem);
   uint64_t *seg = reinterpret_cast<uint64_t *>(smem + (sizeof(amp_t<R>) << P.t));
-  fill_segments(P, seg);
+  fill_segments<NT>(P, seg);
   uint64_t tix = blockIdx.x;
   if (tix >= n_tiles) return;  // uniform over the workgroup
   const uint32_t t = P.t, L = P.L;
@@ -751,7 +754,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
   };
   if (next < n_tiles) prefetch(nbase);
   while (true) {
-    run_stages<R, RB, 2>(tile, t, base, P.n_stages, stages, rops, tops, mats);
+    run_stages<R, RB, 2, NT>(tile, t, base, P.n_stages, stages, rops, tops, mats);
     uint32_t ls = l0;
     asm volatile("" : "+v"(ls));
 #pragma unroll
@@ -951,6 +954,14 @@ __global__ __launch_bounds__(kBlock) void k_drop_bit(const amp_t<R> *__restrict_
 }
 
 // ------------------------------------------------------------------------------ Pauli strings
+// Pauli strings in expectation values are Hermitian (the caller drops the others, whose real part
+// is zero): for x != 0 the amplitudes i and j = i ^ x contribute complex-conjugate terms to
+// <psi|omega X^x Z^z|psi>, so each pair is visited once, from the i with bit h (the highest bit of
+// x) clear, and counted twice.  That reads every amplitude once per string instead of twice.
+
+// partials[2b + {0,1}] <- block sums of f (-1)^popcount(j & z) conj(psi_i) psi_j, j = i ^ x, over the
+// i with bit h clear, f = 2 (x != 0); over every i with f = 1 (x = 0).  omega times the total has
+// the real part of <psi|omega X^x Z^z|psi>.
 template <typename R>
 __global__ __launch_bounds__(kBlock) void k_pauli_expect(const amp_t<R> *__restrict__ psi,
                                                          uint64_t n, uint64_t x, uint64_t z,
@@ -958,12 +969,23 @@ __global__ __launch_bounds__(kBlock) void k_pauli_expect(const amp_t<R> *__restr
   __shared__ double lds[kBlock / 64];
   Kahan sr, si;
   const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
-    const uint64_t j = i ^ x;
-    amp_t<R> a = psi[i], b = psi[j];
-    double s = (__popcll(j & z) & 1) ? -1.0 : 1.0;
-    sr.add(s * (re(a) * re(b) + im(a) * im(b)));
-    si.add(s * (re(a) * im(b) - im(a) * re(b)));
+  if (x == 0) {
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+      const amp_t<R> a = psi[i];
+      const double v = (double)re(a) * re(a) + (double)im(a) * im(a);
+      sr.add((__popcll(i & z) & 1) ? -v : v);
+    }
+  } else {
+    const int hb = 63 - __clzll(x);
+    const uint64_t lo = (uint64_t(1) << hb) - 1;
+    for (uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; g < (n >> 1); g += stride) {
+      const uint64_t i = ((g & ~lo) << 1) | (g & lo);
+      const uint64_t j = i ^ x;
+      const amp_t<R> a = psi[i], b = psi[j];
+      const double f = (__popcll(j & z) & 1) ? -2.0 : 2.0;
+      sr.add(f * ((double)re(a) * re(b) + (double)im(a) * im(b)));
+      si.add(f * ((double)re(a) * im(b) - (double)im(a) * re(b)));
+    }
   }
   double tr = block_sum(sr.value(), lds);
   double ti = block_sum(si.value(), lds);
@@ -973,30 +995,56 @@ __global__ __launch_bounds__(kBlock) void k_pauli_expect(const amp_t<R> *__restr
   }
 }
 
+// All strings in one pass over a state that fits the caches: partials[b] <- block sums of
+// Re(conj(psi_i) u_i), u_i = sum_t f_t c_t (-1)^popcount(j_t & z_t) psi_{j_t}, j_t = i ^ x_t, where a
+// string with x_t != 0 counts only at the i with bit h_t clear, with f_t = 2 (f_t = 1 for x_t = 0).
+// Partner loads are issued kInFlight strings at a time.
 template <typename R>
 __global__ __launch_bounds__(kBlock) void k_pauli_energy(const amp_t<R> *__restrict__ psi, uint64_t n,
                                                          const PauliTerm *__restrict__ terms,
                                                          int n_terms, double *partials) {
+  constexpr int kInFlight = 8;
   __shared__ double lds[kBlock / 64];
   Kahan acc;
   const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
   for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
     const amp_t<R> a = psi[i];
-    const double ar = re(a), ai = im(a);
-    double ur = 0.0, ui = 0.0;
-    for (int t = 0; t < n_terms; ++t) {
-      const PauliTerm pt = terms[t];
-      const uint64_t j = i ^ pt.x;
-      const amp_t<R> b = psi[j];
+    double ur1 = 0.0, ui1 = 0.0, ur2 = 0.0, ui2 = 0.0;  // strings with x = 0, paired strings
+    auto term = [&](const PauliTerm &pt, const amp_t<R> &b) {
       double br = re(b), bi = im(b);
-      if (__popcll(j & pt.z) & 1) {
+      if (__popcll((i ^ pt.x) & pt.z) & 1) {
         br = -br;
         bi = -bi;
       }
-      ur += br * pt.c_re - bi * pt.c_im;
-      ui += br * pt.c_im + bi * pt.c_re;
+      const double vr = br * pt.c_re - bi * pt.c_im, vi = br * pt.c_im + bi * pt.c_re;
+      if (pt.x) {
+        ur2 += vr;
+        ui2 += vi;
+      } else {
+        ur1 += vr;
+        ui1 += vi;
+      }
+    };
+    int t = 0;
+    for (; t + kInFlight <= n_terms; t += kInFlight) {
+      amp_t<R> b[kInFlight];
+      bool on[kInFlight];
+#pragma unroll
+      for (int u = 0; u < kInFlight; ++u) {
+        const uint64_t x = terms[t + u].x;
+        on[u] = x == 0 || !((i >> (63 - __clzll(x))) & 1);
+        if (on[u]) b[u] = psi[i ^ x];
+      }
+#pragma unroll
+      for (int u = 0; u < kInFlight; ++u)
+        if (on[u]) term(terms[t + u], b[u]);
     }
-    acc.add(ar * ur + ai * ui);  // Re(conj(a) * u)
+    for (; t < n_terms; ++t) {
+      const PauliTerm &pt = terms[t];
+      if (pt.x && ((i >> (63 - __clzll(pt.x))) & 1)) continue;
+      term(pt, psi[i ^ pt.x]);
+    }
+    acc.add((double)re(a) * (ur1 + 2.0 * ur2) + (double)im(a) * (ui1 + 2.0 * ui2));  // Re(conj(a) * u)
   }
   double t = block_sum(acc.value(), lds);
   if (threadIdx.x == 0) partials[blockIdx.x] = t;
@@ -1299,10 +1347,14 @@ void launch_tile_pass(amp_t<R> *psi, uint64_t n_tiles, const TilePass &P, const 
   const uint64_t chunk = uint64_t(kStreamChunk) * kBlock * Vec16<R>::n;  // amplitudes streamed per round
   const uint64_t tsize = uint64_t(1) << P.t;
   bool done = false;
-  if constexpr (RB == 4 && sizeof(R) == 4) {  // the only instantiation of the pipelined kernel
+  if constexpr (RB == 4 && sizeof(R) == 4) {  // the only instantiations of the pipelined kernel
     if (prefetch && tsize == chunk) {
-      hipLaunchKernelGGL((k_tile_pass_pf<R, RB>), dim3((unsigned)grid), dim3(kBlock), lds, s, psi, n_tiles, P,
+      hipLaunchKernelGGL((k_tile_pass_pf<R, RB, kBlock>), dim3((unsigned)grid), dim3(kBlock), lds, s, psi, n_tiles, P,
                          d_stages, d_rops, d_tops, d_mats);
+      done = true;
+    } else if (prefetch && tsize == 2 * chunk) {
+      hipLaunchKernelGGL((k_tile_pass_pf<R, RB, 2 * kBlock>), dim3((unsigned)grid), dim3(2 * kBlock), lds, s, psi,
+                         n_tiles, P, d_stages, d_rops, d_tops, d_mats);
       done = true;
     }
   }
@@ -1341,7 +1393,8 @@ void prepare_tile_kernels() {
   QS_HIP(hipFuncSetAttribute((const void *)k_tile_pass<float, 5, false>, hipFuncAttributeMaxDynamicSharedMemorySize, kTileLdsMax<float>));
   QS_HIP(hipFuncSetAttribute((const void *)k_tile_pass<double, 4, false>, hipFuncAttributeMaxDynamicSharedMemorySize, kTileLdsMax<double>));
   QS_HIP(hipFuncSetAttribute((const void *)k_tile_pass<double, 5, false>, hipFuncAttributeMaxDynamicSharedMemorySize, kTileLdsMax<double>));
-  QS_HIP(hipFuncSetAttribute((const void *)k_tile_pass_pf<float, 4>, hipFuncAttributeMaxDynamicSharedMemorySize, kTileLdsMax<float>));
+  QS_HIP(hipFuncSetAttribute((const void *)k_tile_pass_pf<float, 4, kBlock>, hipFuncAttributeMaxDynamicSharedMemorySize, kTileLdsMax<float>));
+  QS_HIP(hipFuncSetAttribute((const void *)k_tile_pass_pf<float, 4, 2 * kBlock>, hipFuncAttributeMaxDynamicSharedMemorySize, kTileLdsMax<float>));
   QS_HIP(hipFuncSetAttribute((const void *)k_batch_tile_pass<float, 4>, hipFuncAttributeMaxDynamicSharedMemorySize, kTileLdsMax<float>));
   QS_HIP(hipFuncSetAttribute((const void *)k_batch_tile_pass<float, 5>, hipFuncAttributeMaxDynamicSharedMemorySize, kTileLdsMax<float>));
   QS_HIP(hipFuncSetAttribute((const void *)k_batch_tile_pass<double, 4>, hipFuncAttributeMaxDynamicSharedMemorySize, kTileLdsMax<double>));

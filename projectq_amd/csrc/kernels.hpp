@@ -2,6 +2,8 @@
 // synchronises.  Index conventions: amplitude index bit p belongs to the qubit at position p.
 #pragma once
 
+#include <complex>
+
 #include "common.hpp"
 
 namespace qs {
@@ -15,6 +17,14 @@ struct PauliTerm {
   uint64_t x, z;
   double c_re, c_im;
 };
+
+// omega X^x Z^z (omega one of 1, -1, i, -i) is Hermitian iff omega^2 = (-1)^popcount(x & z); the
+// product of Paulis on distinct qubits always is, a string naming a qubit twice may be
+// anti-Hermitian, and then Re<psi|P|psi> = 0.
+inline bool pauli_hermitian(uint64_t x, uint64_t z, std::complex<double> omega) {
+  const double sigma = (__builtin_popcountll(x & z) & 1) ? -1.0 : 1.0;
+  return (omega * omega).real() * sigma > 0.0;
+}
 
 // Register layout and constants of one emulate_math call.
 constexpr int kMaxRegs = 8;
@@ -100,12 +110,14 @@ template <typename R>
 void drop_bit(const amp_t<R> *src, amp_t<R> *dst, uint64_t n_out, int p, int value,
               hipStream_t s);
 
-// partials[2*b + {0,1}] <- block sums of conj(psi_i) * (-1)^popcount((i^x)&z) * psi_{i^x}
+// Both expectation kernels take Hermitian strings only, omega X^x Z^z with omega^2 =
+// (-1)^popcount(x & z) (pauli_hermitian); the caller drops the others (their real part is zero).
+// partials[2*b + {0,1}] <- block sums whose total S satisfies Re(omega S) = Re<psi|omega X^x Z^z|psi>
 template <typename R>
 void pauli_expect(const amp_t<R> *psi, uint64_t n_amps, uint64_t x, uint64_t z,
                   double *partials, hipStream_t s);
-// partials[b] <- block sums of Re( conj(psi_i) * sum_t c_t (-1)^popcount((i^x_t)&z_t) psi_{i^x_t} )
-// (all terms in one pass; c_t already carries the string's phase omega_t)
+// partials[b] <- block sums of sum_t Re<psi|c_t X^x_t Z^z_t|psi> (all terms in one pass; c_t
+// already carries the string's phase omega_t)
 template <typename R>
 void pauli_energy(const amp_t<R> *psi, uint64_t n_amps, const PauliTerm *terms, int n_terms,
                   double *partials, hipStream_t s);

@@ -108,6 +108,10 @@ double expectation_noalloc(const void *state, unsigned n_qubits, bool c64, const
       const size_t m = std::min(kChunk, n_terms - t0);
       for (size_t i = 0; i < m; ++i) {
         encode(t0 + i, term_idx, term_ops, term_offsets, positions, n_positions, &pt[i], &omega[i]);
+        if (!dev::pauli_hermitian(pt[i].x, pt[i].z, omega[i])) {
+          pt[i] = dev::PauliTerm{0, 0, 0.0, 0.0};  // Re<psi|P|psi> = 0 for an anti-Hermitian string
+          continue;
+        }
         const std::complex<double> c = coeffs[t0 + i] * omega[i];
         pt[i].c_re = c.real();
         pt[i].c_im = c.imag();
@@ -130,6 +134,7 @@ double expectation_noalloc(const void *state, unsigned n_qubits, bool c64, const
     for (size_t i = 0; i < m; ++i) {
       encode(t0 + i, term_idx, term_ops, term_offsets, positions, n_positions, &pt[i], &omega[i]);
       double *out = d_results + 2 * (t0 + i);
+      if (!dev::pauli_hermitian(pt[i].x, pt[i].z, omega[i])) continue;  // contributes 0, read back unused
       if (pt[i].x == 0 && pt[i].z == 0) {
         if (c64)
           dev::norm_masked<float>(static_cast<const float2 *>(state), n_amps, none, 0, d_partials, s);
@@ -148,7 +153,9 @@ double expectation_noalloc(const void *state, unsigned n_qubits, bool c64, const
     QS_HIP(hipMemcpy(res, d_results + 2 * t0, 2 * m * sizeof(double), hipMemcpyDeviceToHost));
     for (size_t i = 0; i < m; ++i) {
       double delta;
-      if (pt[i].x == 0 && pt[i].z == 0)
+      if (!dev::pauli_hermitian(pt[i].x, pt[i].z, omega[i]))
+        delta = 0.0;
+      else if (pt[i].x == 0 && pt[i].z == 0)
         delta = omega[i].real() * res[2 * i];
       else
         delta = (omega[i] * std::complex<double>(res[2 * i], res[2 * i + 1])).real();

@@ -50,7 +50,8 @@ def test_expectation_on_large_states(precision, n):
     h = Simulator(3, precision=precision, capacity_qubits=n)
     commands.execute(h, cmds, gate_fusion=False)
     terms = circuits.random_pauli_operator(n, 10, 62)
-    terms += [([], 0.75), ([], -0.25), ([(0, "Y"), (n - 1, "Y")], 0.5), ([(3, "Z")], 1.25)]
+    terms += [([], 0.75), ([], -0.25), ([(0, "Y"), (n - 1, "Y")], 0.5), ([(3, "Z")], 1.25),
+              ([(n - 1, "X")], 0.6), ([(0, "X")], -0.4), ([(5, "X"), (5, "Y")], 0.8)]
     ids = list(range(n))
     rng = np.random.default_rng(63)
     perm = [int(q) for q in rng.permutation(n)]  # local index -> qubit id not the identity

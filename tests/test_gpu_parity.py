@@ -178,6 +178,12 @@ def test_expectation_and_operator():
     ids = list(range(n))
     eo, eh = o.get_expectation_value(terms, ids), h.get_expectation_value(terms, ids)
     assert abs(eo - eh) <= 1e-12 * max(1.0, abs(eo))
+    # strings naming a qubit twice: X.X is the identity; X.Y and Y.Z.X are anti-Hermitian, whose
+    # real part is zero (the kernels pair i with i ^ x only for Hermitian strings)
+    odd = [([(2, "X"), (2, "X")], 0.4), ([(2, "X"), (2, "Y")], 0.3), ([(4, "Y"), (4, "Z"), (11, "X")], -0.7),
+           ([(11, "Y"), (0, "X"), (11, "Y")], 0.9)]
+    eo, eh = o.get_expectation_value(terms + odd, ids), h.get_expectation_value(terms + odd, ids)
+    assert abs(eo - eh) <= 1e-12 * max(1.0, abs(eo))
     cterms = [(t, c * (1 + 0.5j)) for t, c in terms[:10]]
     o.apply_qubit_operator(cterms, ids)
     h.apply_qubit_operator(cterms, ids)
@@ -261,6 +267,49 @@ def test_pipelined_tile_pass_26q_c64():
         bits = [(int(idx) >> q) & 1 for q in range(n)]
         a, b = ref.get_amplitude(bits, list(range(n))), tile.get_amplitude(bits, list(range(n)))
         assert abs(a - b) <= 1e-4 * 2 ** (-n / 2)  # typical |amplitude| is 2^(-n/2)
+
+
+@pytest.mark.parametrize("rule", ["lookahead", "earliest"])
+def test_tile_pass_2p13_tiles_28q_c64(rule):
+    """28 qubits, complex64, tiles of 2^13 amplitudes: the pipelined pass with 512-thread
+    workgroups (eight 16-byte vectors per thread), 2^15 tiles on 8192 workgroups, so every
+    workgroup pipelines four tiles.  Checked against the unfused complex128 run of the same gates,
+    with 128-byte and 256-byte segments."""
+    from projectq_amd import Simulator
+
+    n = 28
+    print(f"tile pass 2^13 c64 n={n} rule={rule}", flush=True)
+    gates = [c for c in circuits.random_layers(n, 6, n // 2, 47, measure=False) if c[0] == "gate"]
+
+    def run(tile_shape=None, **opts):
+        sim = Simulator(5, capacity_qubits=n, **opts)
+        if tile_shape:
+            sim.set_tile_params(*tile_shape)
+            sim.set_slot_rule(rule)
+        for q in range(n):
+            sim.allocate_qubit(q)
+        window = opts.get("fusion") == "window"
+        for _, m, t, c in gates:
+            sim.apply_controlled_gate(m, list(t), list(c))
+            if not window:
+                sim.run()
+        sim.run()
+        sim.synchronize()
+        return sim
+
+    ref = run(precision="c128", fusion="reference")
+    for shape in ((13, 4, 4), (13, 4, 5)):
+        tile = run(shape, precision="c64", fusion="window", planner="tile")
+        plan = tile.last_plan()
+        assert (plan["tile_bits"], plan["low_bits"]) == (13, shape[2]), plan
+        inf = 1.0 - abs(ref.overlap(tile)) ** 2 / (ref.overlap(ref).real * tile.overlap(tile).real)
+        assert inf <= 1e-6, (shape, inf)
+        rng = np.random.default_rng(4)
+        for idx in rng.integers(0, 2**n, size=32):
+            bits = [(int(idx) >> q) & 1 for q in range(n)]
+            a, b = ref.get_amplitude(bits, list(range(n))), tile.get_amplitude(bits, list(range(n)))
+            assert abs(a - b) <= 1e-4 * 2 ** (-n / 2), (shape, idx)
+        del tile
 
 
 def test_tile_pass_2p11_tiles_25q_c128():
