@@ -539,7 +539,7 @@ void Engine::plan_window(const std::vector<Gate> &gates, std::vector<Block> *blo
   };
   std::vector<Cand> cands;
   const std::vector<std::array<unsigned, 3>> shapes =
-      c64_ ? std::vector<std::array<unsigned, 3>>{{12, 4, 5}, {12, 4, 6}, {12, 4, 4}, {11, 4, 5}, {10, 4, 4}, {12, 5, 5}, {13, 5, 5}}
+      c64_ ? std::vector<std::array<unsigned, 3>>{{12, 4, 5}, {12, 4, 6}, {12, 4, 4}, {11, 4, 5}, {10, 4, 4}, {12, 5, 5}, {13, 4, 4}, {13, 4, 5}}
            : std::vector<std::array<unsigned, 3>>{{12, 4, 4}, {12, 4, 5}, {11, 4, 4}, {10, 4, 4}, {12, 5, 4}};
   for (int rule = 0; rule < kSlotRules; ++rule)
     for (const auto &s : shapes) cands.push_back(Cand{s[0], s[1], s[2], rule});
@@ -569,12 +569,18 @@ void Engine::plan_window(const std::vector<Gate> &gates, std::vector<Block> *blo
   }
   rows_of.push_back(blocks.size());
   const std::vector<float> y = model_->predict(X, X.size() / kPassFeatures, stream_);
-  size_t r = 0, best = 0;
+  // Every pass streams the whole state, so a plan with several times the passes of another cannot
+  // win by much; such candidates are left out, which keeps a cost model extrapolating on a gate
+  // mix it was not trained on from picking a plan of many cheap-looking passes.
+  size_t fewest = std::max<size_t>(rows_of[0], 1);
+  for (size_t n_rows : rows_of) fewest = std::min(fewest, std::max<size_t>(n_rows, 1));
+  size_t r = 0, best = rows_of.size();
   double best_ms = 0.0;
   for (size_t c = 0; c < rows_of.size(); ++c) {
     double ms = 0.0;
     for (size_t i = 0; i < rows_of[c]; ++i, ++r) ms += pass_time_from_target(y[r], n_);
-    if (c == 0 || ms < best_ms) {
+    if (rows_of[c] > kMaxPassRatio * fewest) continue;
+    if (best == rows_of.size() || ms < best_ms) {
       best_ms = ms;
       best = c;
     }

@@ -159,6 +159,8 @@ class Engine {
   // (PROJECTQ_AMD_MFMA=all)
   int mfma_mode_ = 1;
   static constexpr uint64_t kMfmaMinGroups = 4096;
+  // learned planner: candidates with more than this many times the fewest passes are not scored
+  static constexpr size_t kMaxPassRatio = 3;
   bool use_mfma(int k, const std::vector<uint32_t> &tpos, uint64_t n_groups) const;
   bool recording_ = false;
   struct PassRecord {

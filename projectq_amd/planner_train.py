@@ -2,8 +2,8 @@
 
     python -m projectq_amd.planner_train --circuits 2000 --out projectq_amd/planner_mlp.npz
 
-1. Generate seeded random circuits (24..30 qubits, mixed 1/2/3-qubit gates or random layers with
-   CNOTs), run each once in window mode with a random planner configuration (tile passes of a
+1. Generate seeded random circuits (24..30 qubits: mixed 1/2/3-qubit gates, random layers with
+   CNOTs, or a QFT followed by layers of rotations and multi-controlled Rz), run each once in window mode with a random planner configuration (tile passes of a
    random shape under a random slot rule, or greedy max-5 blocks), recording every pass: its
    features and its GPU time.
 2. Train the 24 -> 64 -> 64 -> 1 MLP on log(time per 2^20 amplitudes) with the HIP training
@@ -23,7 +23,7 @@ from . import circuits
 from ._binding import MLP, PASS_FEATURES
 from ._cppsim import Simulator, library
 
-TILE_SHAPES = {"c64": [(12, 4, 5), (12, 4, 6), (12, 4, 4), (11, 4, 5), (10, 4, 4), (12, 5, 5), (13, 5, 5)],
+TILE_SHAPES = {"c64": [(12, 4, 5), (12, 4, 6), (12, 4, 4), (11, 4, 5), (10, 4, 4), (12, 5, 5), (13, 4, 4), (13, 4, 5)],
                "c128": [(12, 4, 4), (12, 4, 5), (11, 4, 4), (10, 4, 4), (12, 5, 4)]}
 
 DEFAULT_MODEL = os.path.join(os.path.dirname(os.path.abspath(__file__)), "planner_mlp.npz")
@@ -34,9 +34,16 @@ def log(msg):
 
 
 def training_circuit(rng, n):
-    if rng.random() < 0.6:
+    u = rng.random()
+    if u < 0.5:
         return circuits.mixed_circuit(n, int(rng.integers(3, 10)), int(rng.integers(1 << 30)))
-    return circuits.random_layers(n, int(rng.integers(1, 4)), n // 2, int(rng.integers(1 << 30)), measure=False)
+    if u < 0.8:
+        return circuits.random_layers(n, int(rng.integers(1, 4)), n // 2, int(rng.integers(1 << 30)), measure=False)
+    # a QFT followed by rotation layers with multi-controlled Rz gates: long runs of controlled
+    # phases, the diagonal-heavy passes the other two families rarely produce
+    cmds, _ = circuits.qft_rotations_adder(n, int(rng.integers(0, 4)), n // 2, int(rng.integers(1 << 30)),
+                                           n_probabilities=0, measure=False)
+    return cmds
 
 
 def generate(n_circuits, qmin, qmax, seed):
