@@ -28,6 +28,14 @@ BitInsert make_insert(std::vector<uint32_t> positions) {
   return ins;
 }
 
+uint64_t insert_zeros_host(uint64_t g, const BitInsert &ins) {
+  for (uint32_t i = 0; i < ins.count; ++i) {
+    const uint32_t p = ins.pos[i];
+    g = ((g >> p) << (p + 1)) | (g & ((uint64_t(1) << p) - 1));
+  }
+  return g;
+}
+
 std::vector<uint32_t> bits_of(uint64_t mask) {
   std::vector<uint32_t> out;
   for (uint32_t p = 0; p < 64; ++p)
@@ -210,6 +218,8 @@ void Engine::allocate_qubit(unsigned id) {
   if (n_ >= 40) throw ValueError("too many qubits");
   ensure_capacity(amps() * 2);
   QS_HIP(hipMemsetAsync((char *)psi_ + amps() * amp_bytes(), 0, amps() * amp_bytes(), stream_));
+  pointer_out_ = false;
+  subspace_fix(uint64_t(1) << n_, 0);  // the new qubit is |0>: the upper half is zero
   pos_[id] = n_++;
 }
 
@@ -273,6 +283,10 @@ void Engine::deallocate_qubit(unsigned id) {
     if (kv.second > p) kv.second--;
   pos_.erase(id);
   n_--;
+  pointer_out_ = false;
+  const uint64_t low = (uint64_t(1) << p) - 1;
+  zmask_ = (zmask_ & low) | ((zmask_ >> 1) & ~low);
+  zval_ = (zval_ & low) | ((zval_ >> 1) & ~low);
   norm_changed();
 }
 
@@ -285,12 +299,16 @@ std::vector<uint8_t> Engine::measure_qubits(const std::vector<unsigned> &ids) {
   check_known(ids, "measure_qubits");
   const double rnd = uni_(rng_);
   const uint64_t N = amps();
+  // the scan runs over the subspace outside which every amplitude is zero: the same amplitudes in
+  // the same index order, without the zeros
+  const BitInsert sub = make_insert(bits_of(zmask_));
+  const uint64_t n_sub = N >> sub.count;
   uint64_t pick;
   if (!(rnd > 0.0)) {
     pick = ~uint64_t(0);  // the reference's loop never runs and `pick--` wraps
   } else {
-    const int lg = (int)std::min<unsigned>(n_, 12);
-    const uint64_t len = uint64_t(1) << lg, chunks = N >> lg;
+    const int lg = (int)std::min<unsigned>(n_ - sub.count, 12);
+    const uint64_t len = uint64_t(1) << lg, chunks = n_sub >> lg;
     if (chunks > chunks_cap_) {
       (void)hipFree(d_chunks_);
       (void)hipHostFree(h_chunks_);
@@ -302,7 +320,7 @@ std::vector<uint8_t> Engine::measure_qubits(const std::vector<unsigned> &ids) {
     }
     dispatch([&](auto tag) {
       using R = decltype(tag);
-      dev::chunk_norms<R>((const amp_t<R> *)psi_, N, lg, d_chunks_, stream_);
+      dev::chunk_norms<R>((const amp_t<R> *)psi_, n_sub, sub, zval_, lg, d_chunks_, stream_);
     });
     QS_HIP(hipMemcpyAsync(h_chunks_, d_chunks_, chunks * sizeof(double), hipMemcpyDeviceToHost, stream_));
     sync();
@@ -319,11 +337,12 @@ std::vector<uint8_t> Engine::measure_qubits(const std::vector<unsigned> &ids) {
       QS_HIP(hipMemcpyAsync(d_key_, h_key_, sizeof(unsigned long long), hipMemcpyHostToDevice, stream_));
       dispatch([&](auto tag) {
         using R = decltype(tag);
-        dev::chunk_pick<R>((const amp_t<R> *)psi_, c * len, len, run_sum, rnd, d_key_, stream_);
+        dev::chunk_pick<R>((const amp_t<R> *)psi_, sub, zval_, c * len, len, run_sum, rnd, d_key_, stream_);
       });
       QS_HIP(hipMemcpyAsync(h_key_, d_key_, sizeof(unsigned long long), hipMemcpyDeviceToHost, stream_));
       sync();
-      pick = (*h_key_ == ~0ull) ? (c + 1) * len - 1 : (uint64_t)*h_key_;
+      const uint64_t compact = (*h_key_ == ~0ull) ? (c + 1) * len - 1 : (uint64_t)*h_key_;
+      pick = insert_zeros_host(compact, sub) | zval_;
     }
   }
   std::vector<uint8_t> res(ids.size());
@@ -334,12 +353,15 @@ std::vector<uint8_t> Engine::measure_qubits(const std::vector<unsigned> &ids) {
     mask |= uint64_t(1) << p;
     val |= uint64_t(res[i]) << p;
   }
-  double kept = masked_norm(mask, val);
+  // outcome bits that contradict fixed bits select only zero amplitudes (the reference's edge
+  // cases of a draw of 0 or beyond the total)
+  const double kept = ((val ^ zval_) & mask & zmask_) ? 0.0 : masked_norm(mask | zmask_, val | (zval_ & ~mask));
   double scale = 1.0 / std::sqrt(kept);
   dispatch([&](auto tag) {
     using R = decltype(tag);
-    dev::collapse<R>((amp_t<R> *)psi_, N, mask, val, scale, stream_);
+    dev::collapse<R>((amp_t<R> *)psi_, n_sub, sub, zval_, mask, val, scale, stream_);
   });
+  subspace_fix(mask, val);
   norm_changed();
   return res;
 }
@@ -351,6 +373,23 @@ void Engine::apply_controlled_gate(Gate g) {
   check_known(g.ctrl, "apply_controlled_gate");
   if (g.m.size() != (size_t(1) << (2 * g.ids.size())))
     throw ValueError("apply_controlled_gate: matrix size does not match the number of qubits");
+  pointer_out_ = false;
+  {
+    // a non-diagonal gate can move amplitudes along its targets: they leave the known-zero
+    // description (controls and diagonal gates keep zero amplitudes zero)
+    const size_t d = size_t(1) << g.ids.size();
+    bool diagonal = true;
+    for (size_t r = 0; r < d && diagonal; ++r)
+      for (size_t c = 0; c < d; ++c)
+        if (r != c && g.m[r * d + c] != cplx(0.0, 0.0)) {
+          diagonal = false;
+          break;
+        }
+    if (!diagonal) {
+      zmask_ &= ~mask_of(g.ids);
+      zval_ &= zmask_;
+    }
+  }
   // the complex64 norm guard must know whether a non-unitary gate is among the gates of the next
   // execution, and how far their FP32 matrices may move the norm
   bool nonunitary = false;
@@ -846,6 +885,7 @@ dev::MathArgs Engine::math_args(const std::vector<std::vector<unsigned>> &regs,
 }
 
 void Engine::math_apply(dev::MathArgs args, const int32_t *d_table, bool known_bijective) {
+  subspace_unknown();  // amplitudes may move anywhere
   const uint64_t N = amps();
   QS_HIP(hipMemsetAsync(d_flags_, 0, sizeof(int) * 2, stream_));
   bool injective = known_bijective;
@@ -876,6 +916,7 @@ void Engine::math_apply(dev::MathArgs args, const int32_t *d_table, bool known_b
   if (h_flags_[0]) throw RuntimeError("emulate_math: a register value does not fit a C++ int");
   std::swap(psi_, buf_[0]);
   norm_changed();
+  subspace_unknown();
 }
 
 // simulator.hpp:224-290 for the opcodes of include/projectq_amd.h
@@ -999,6 +1040,7 @@ double Engine::get_expectation_value(const std::vector<PauliString> &terms,
 // simulator.hpp:324-350: psi <- sum_t c_t P_t psi in one out-of-place pass.
 void Engine::apply_qubit_operator(const std::vector<PauliString> &terms, const std::vector<cplx> &coeffs,
                                   const std::vector<unsigned> &ids) {
+  subspace_unknown();  // amplitudes may move anywhere
   run();
   check_known(ids, "apply_qubit_operator");
   std::vector<dev::PauliTerm> pt(terms.size());
@@ -1019,6 +1061,7 @@ void Engine::apply_qubit_operator(const std::vector<PauliString> &terms, const s
   });
   std::swap(psi_, buf_[0]);
   norm_changed();
+  subspace_unknown();
 }
 
 // simulator.hpp:386-438: scaled, truncated Taylor series of exp(-i H t) on the controlled
@@ -1026,6 +1069,7 @@ void Engine::apply_qubit_operator(const std::vector<PauliString> &terms, const s
 void Engine::emulate_time_evolution(const std::vector<PauliString> &terms, const std::vector<double> &coeffs,
                                     double time, const std::vector<unsigned> &ids,
                                     const std::vector<unsigned> &ctrl) {
+  subspace_unknown();  // amplitudes may move anywhere
   run();
   check_known(ids, "emulate_time_evolution");
   check_known(ctrl, "emulate_time_evolution");
@@ -1074,6 +1118,7 @@ void Engine::emulate_time_evolution(const std::vector<PauliString> &terms, const
   }
   sync();
   norm_changed();
+  subspace_unknown();
 }
 
 // ------------------------------------------------------------------------------ queries
@@ -1119,6 +1164,7 @@ cplx Engine::get_amplitude(const std::vector<uint8_t> &bits, const std::vector<u
 
 // simulator.hpp:440-454
 void Engine::set_wavefunction(const double *wf, size_t n_amps, const std::vector<unsigned> &ordering) {
+  subspace_unknown();  // amplitudes may move anywhere
   run();
   bool ok = pos_.size() == ordering.size();
   for (unsigned q : ordering) ok = ok && pos_.count(q);
@@ -1139,6 +1185,7 @@ void Engine::set_wavefunction(const double *wf, size_t n_amps, const std::vector
     sync();
   }
   norm_changed();
+  subspace_unknown();
 }
 
 // simulator.hpp:456-485
@@ -1156,13 +1203,15 @@ void Engine::collapse_wavefunction(const std::vector<unsigned> &ids, const std::
     val |= uint64_t(values[i] ? 1 : 0) << pos_[ids[i]];
   }
   val &= mask;
-  const double p = masked_norm(mask, val);
+  const double p = ((val ^ zval_) & mask & zmask_) ? 0.0 : masked_norm(mask | zmask_, val | (zval_ & ~mask));
   if (p < 1.e-12) throw RuntimeError("collapse_wavefunction(): Invalid collapse! Probability is ~0.");
   const double scale = 1. / std::sqrt(p);
+  const BitInsert sub = make_insert(bits_of(zmask_));
   dispatch([&](auto tag) {
     using R = decltype(tag);
-    dev::collapse<R>((amp_t<R> *)psi_, amps(), mask, val, scale, stream_);
+    dev::collapse<R>((amp_t<R> *)psi_, amps() >> sub.count, sub, zval_, mask, val, scale, stream_);
   });
+  subspace_fix(mask, val);
   norm_changed();
 }
 
@@ -1204,6 +1253,8 @@ void Engine::device_state(void **ptr, uint64_t *n_amps, int *precision) {
   *n_amps = amps();
   *precision = c64_ ? 1 : 0;
   norm_changed();  // the caller may write through the pointer
+  subspace_unknown();
+  pointer_out_ = true;
 }
 
 void Engine::qubit_map(uint32_t *ids, uint32_t *positions, size_t capacity, size_t *n_qubits) {
@@ -1380,10 +1431,12 @@ void Engine::load_state(const char *path) {
   }
   sync();
   norm_changed();
+  subspace_unknown();
 }
 
 void Engine::set_wavefunction_device(const void *src, uint64_t n_amps, int src_precision,
                                      const std::vector<unsigned> &ordering) {
+  subspace_unknown();  // amplitudes may move anywhere
   run();
   bool ok = pos_.size() == ordering.size();
   for (unsigned q : ordering) ok = ok && pos_.count(q);
@@ -1407,6 +1460,7 @@ void Engine::set_wavefunction_device(const void *src, uint64_t n_amps, int src_p
   }
   sync();
   norm_changed();
+  subspace_unknown();
 }
 
 }  // namespace qs

@@ -140,6 +140,20 @@ class Engine {
   }
   void before_unitary_passes();
   void after_passes();
+  // Known-zero amplitudes: every amplitude i with (i & zmask_) != zval_ is exactly zero (qubits
+  // fixed by a measurement or a collapse, or freshly allocated and not touched by a non-diagonal
+  // gate since).  Measurements then read and write only that subspace: after All(Measure) the k-th
+  // qubit's measurement touches 2^(n-k+1) amplitudes.  A non-diagonal gate frees its targets; an
+  // operation that can move amplitudes anywhere, or a handed-out device pointer, frees all bits.
+  uint64_t zmask_ = 0, zval_ = 0;
+  bool pointer_out_ = false;  // device_state handed the state out; it may be written until the
+                              // next call that applies gates, allocates or frees qubits
+  void subspace_unknown() { zmask_ = zval_ = 0; }
+  void subspace_fix(uint64_t mask, uint64_t val) {
+    if (pointer_out_) return;
+    zmask_ |= mask;
+    zval_ = (zval_ & ~mask) | (val & mask);
+  }
   void plan_window(const std::vector<Gate> &gates, std::vector<Block> *blocks, std::vector<TileProgram> *progs,
                    unsigned *rb, PlanInfo *info);
   unsigned tile_tmax_ = 13, tile_rb_ = 5;  // tile shape limits (PROJECTQ_AMD_TILE="t:rb" overrides)

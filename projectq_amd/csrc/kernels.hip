@@ -835,11 +835,15 @@ __global__ void k_finish_sum(const double *__restrict__ partials, int n_partials
   out[j] = t;
 }
 
+// The measurement kernels run over a subspace: compact index c is the amplitude
+// insert_zeros(c, sub) | sval, so the bits listed in `sub` are fixed to their values in sval (the
+// caller knows every amplitude outside is zero).  With sub.count = 0 they cover the whole state.
 template <typename R>
-__global__ __launch_bounds__(kBlock) void k_collapse(amp_t<R> *__restrict__ psi, uint64_t n,
-                                                     uint64_t mask, uint64_t val, double scale) {
+__global__ __launch_bounds__(kBlock) void k_collapse(amp_t<R> *__restrict__ psi, uint64_t n_sub, BitInsert sub,
+                                                     uint64_t sval, uint64_t mask, uint64_t val, double scale) {
   const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+  for (uint64_t c = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; c < n_sub; c += stride) {
+    const uint64_t i = insert_zeros(c, sub) | sval;
     amp_t<R> a = psi[i];
     if ((i & mask) != val) a = make_amp<R>(0.0, 0.0);
     else a = make_amp<R>(re(a) * scale, im(a) * scale);
@@ -848,14 +852,14 @@ __global__ __launch_bounds__(kBlock) void k_collapse(amp_t<R> *__restrict__ psi,
 }
 
 template <typename R>
-__global__ __launch_bounds__(kBlock) void k_chunk_norms(const amp_t<R> *__restrict__ psi,
-                                                        int chunk_log2, double *sums) {
+__global__ __launch_bounds__(kBlock) void k_chunk_norms(const amp_t<R> *__restrict__ psi, BitInsert sub,
+                                                        uint64_t sval, int chunk_log2, double *sums) {
   __shared__ double lds[kBlock / 64];
   const uint64_t base = (uint64_t)blockIdx.x << chunk_log2;
   const uint64_t len = uint64_t(1) << chunk_log2;
   double acc = 0.0;
   for (uint64_t j = threadIdx.x; j < len; j += blockDim.x) {
-    amp_t<R> a = psi[base + j];
+    amp_t<R> a = psi[insert_zeros(base + j, sub) | sval];
     double x = re(a), y = im(a);
     acc = fma(x, x, fma(y, y, acc));
   }
@@ -863,17 +867,18 @@ __global__ __launch_bounds__(kBlock) void k_chunk_norms(const amp_t<R> *__restri
   if (threadIdx.x == 0) sums[blockIdx.x] = t;
 }
 
-// One workgroup: thread t owns the contiguous slice [t*per, (t+1)*per) of the chunk.
+// One workgroup: thread t owns the contiguous slice [t*per, (t+1)*per) of the chunk (compact
+// indices of the subspace); the picked compact index goes to *pick.
 template <typename R>
-__global__ __launch_bounds__(kBlock) void k_chunk_pick(const amp_t<R> *__restrict__ psi,
-                                                       uint64_t start, uint64_t len, double base,
+__global__ __launch_bounds__(kBlock) void k_chunk_pick(const amp_t<R> *__restrict__ psi, BitInsert sub,
+                                                       uint64_t sval, uint64_t start, uint64_t len, double base,
                                                        double rnd, unsigned long long *pick) {
   __shared__ double part[kBlock];
   const uint64_t per = (len + kBlock - 1) / kBlock;
   const uint64_t lo = min<uint64_t>(len, threadIdx.x * per), hi = min<uint64_t>(len, lo + per);
   double mine = 0.0;
   for (uint64_t j = lo; j < hi; ++j) {
-    amp_t<R> a = psi[start + j];
+    amp_t<R> a = psi[insert_zeros(start + j, sub) | sval];
     double x = re(a), y = im(a);
     mine += x * x + y * y;
   }
@@ -890,7 +895,7 @@ __global__ __launch_bounds__(kBlock) void k_chunk_pick(const amp_t<R> *__restric
   __syncthreads();
   double run = part[threadIdx.x];
   for (uint64_t j = lo; j < hi; ++j) {
-    amp_t<R> a = psi[start + j];
+    amp_t<R> a = psi[insert_zeros(start + j, sub) | sval];
     double x = re(a), y = im(a);
     run += x * x + y * y;
     if (run >= rnd) {
@@ -1444,23 +1449,26 @@ void finish_sum(const double *partials, int n_partials, int width, double *out, 
 }
 
 template <typename R>
-void collapse(amp_t<R> *psi, uint64_t n, uint64_t mask, uint64_t val, double scale, hipStream_t s) {
-  hipLaunchKernelGGL(k_collapse<R>, dim3(grid_for(n)), dim3(kBlock), 0, s, psi, n, mask, val, scale);
+void collapse(amp_t<R> *psi, uint64_t n_sub, const BitInsert &sub, uint64_t sval, uint64_t mask, uint64_t val,
+              double scale, hipStream_t s) {
+  hipLaunchKernelGGL(k_collapse<R>, dim3(grid_for(n_sub)), dim3(kBlock), 0, s, psi, n_sub, sub, sval, mask, val,
+                     scale);
   QS_HIP(hipGetLastError());
 }
 
 template <typename R>
-void chunk_norms(const amp_t<R> *psi, uint64_t n, int chunk_log2, double *sums, hipStream_t s) {
-  uint64_t chunks = n >> chunk_log2;
-  hipLaunchKernelGGL(k_chunk_norms<R>, dim3((unsigned)chunks), dim3(kBlock), 0, s, psi, chunk_log2,
+void chunk_norms(const amp_t<R> *psi, uint64_t n_sub, const BitInsert &sub, uint64_t sval, int chunk_log2,
+                 double *sums, hipStream_t s) {
+  uint64_t chunks = n_sub >> chunk_log2;
+  hipLaunchKernelGGL(k_chunk_norms<R>, dim3((unsigned)chunks), dim3(kBlock), 0, s, psi, sub, sval, chunk_log2,
                      sums);
   QS_HIP(hipGetLastError());
 }
 
 template <typename R>
-void chunk_pick(const amp_t<R> *psi, uint64_t start, uint64_t len, double base, double rnd,
-                unsigned long long *pick, hipStream_t s) {
-  hipLaunchKernelGGL(k_chunk_pick<R>, dim3(1), dim3(kBlock), 0, s, psi, start, len, base, rnd, pick);
+void chunk_pick(const amp_t<R> *psi, const BitInsert &sub, uint64_t sval, uint64_t start, uint64_t len, double base,
+                double rnd, unsigned long long *pick, hipStream_t s) {
+  hipLaunchKernelGGL(k_chunk_pick<R>, dim3(1), dim3(kBlock), 0, s, psi, sub, sval, start, len, base, rnd, pick);
   QS_HIP(hipGetLastError());
 }
 
@@ -1605,10 +1613,12 @@ template void convert_amps<float, double>(const amp_t<float> *, amp_t<double> *,
                                const uint32_t *, int, const amp_t<R> *, hipStream_t);             \
   template void norm_masked<R>(const amp_t<R> *, uint64_t, const BitInsert &, uint64_t, double *, \
                                hipStream_t);                                                      \
-  template void collapse<R>(amp_t<R> *, uint64_t, uint64_t, uint64_t, double, hipStream_t);      \
-  template void chunk_norms<R>(const amp_t<R> *, uint64_t, int, double *, hipStream_t);           \
-  template void chunk_pick<R>(const amp_t<R> *, uint64_t, uint64_t, double, double,               \
-                              unsigned long long *, hipStream_t);                                 \
+  template void collapse<R>(amp_t<R> *, uint64_t, const BitInsert &, uint64_t, uint64_t, uint64_t, double,    \
+                            hipStream_t);                                                         \
+  template void chunk_norms<R>(const amp_t<R> *, uint64_t, const BitInsert &, uint64_t, int, double *,        \
+                               hipStream_t);                                                      \
+  template void chunk_pick<R>(const amp_t<R> *, const BitInsert &, uint64_t, uint64_t, uint64_t, double,      \
+                              double, unsigned long long *, hipStream_t);                         \
   template void classical_flags<R>(const amp_t<R> *, uint64_t, int, double, int *, hipStream_t);  \
   template void first_pair<R>(const amp_t<R> *, uint64_t, int, double, unsigned long long *,      \
                               hipStream_t);                                                       \

@@ -81,20 +81,24 @@ void norm_masked(const amp_t<R> *psi, uint64_t n_sub, const BitInsert &ins, uint
 // out[0..width) <- sum over blocks of partials[.. * width + j], fixed order (deterministic)
 void finish_sum(const double *partials, int n_partials, int width, double *out, hipStream_t s);
 
-// psi_i <- 0 where (i & mask) != val, psi_i * scale elsewhere
-template <typename R>
-void collapse(amp_t<R> *psi, uint64_t n_amps, uint64_t mask, uint64_t val, double scale,
-              hipStream_t s);
+// The three measurement kernels cover a subspace of the state: compact index c < n_sub stands for
+// amplitude insert_zeros(c, sub) | sval (the bits in `sub` fixed to their values in sval; the
+// caller knows every amplitude outside the subspace is zero).  sub.count = 0, sval = 0: the state.
 
-// sums[c] <- sum |psi_i|^2 over chunk c of 2^chunk_log2 contiguous amplitudes
+// psi_i <- 0 where (i & mask) != val, psi_i * scale elsewhere, for i in the subspace
 template <typename R>
-void chunk_norms(const amp_t<R> *psi, uint64_t n_amps, int chunk_log2, double *sums,
-                 hipStream_t s);
-// *pick <- first index j in [start, start+len) with base + sum_{start<=i<=j} |psi_i|^2 >= rnd
-// (left at its initial value if none)
+void collapse(amp_t<R> *psi, uint64_t n_sub, const BitInsert &sub, uint64_t sval, uint64_t mask, uint64_t val,
+              double scale, hipStream_t s);
+
+// sums[c] <- sum |psi_i|^2 over chunk c of 2^chunk_log2 consecutive compact indices
 template <typename R>
-void chunk_pick(const amp_t<R> *psi, uint64_t start, uint64_t len, double base, double rnd,
-                unsigned long long *pick, hipStream_t s);
+void chunk_norms(const amp_t<R> *psi, uint64_t n_sub, const BitInsert &sub, uint64_t sval, int chunk_log2,
+                 double *sums, hipStream_t s);
+// *pick <- first compact index j in [start, start+len) with base + sum_{start<=c<=j} |psi_c|^2 >=
+// rnd (left at its initial value if none)
+template <typename R>
+void chunk_pick(const amp_t<R> *psi, const BitInsert &sub, uint64_t sval, uint64_t start, uint64_t len, double base,
+                double rnd, unsigned long long *pick, hipStream_t s);
 
 // flags[0] |= any |psi_i|^2 > tol with bit p clear; flags[1] likewise with bit p set
 template <typename R>

@@ -133,6 +133,71 @@ def _superposition(sim, n, seed):
     sim.set_wavefunction(wf, list(range(n)))
 
 
+@pytest.mark.parametrize("precision", ["c128", "c64"])
+def test_measurements_on_known_zero_subspaces(precision):
+    """Measurements after earlier measurements scan only the amplitudes that can be non-zero.  The
+    sequence mixes everything that changes which amplitudes are known zero: one-qubit measurements
+    in ProjectQ's All(Measure) order, a multi-qubit measurement, a diagonal gate (keeps the
+    measured qubits fixed), a non-diagonal gate on a measured qubit (frees it), a controlled gate
+    whose control is measured, collapse_wavefunction, allocation of a fresh qubit (fixed to 0),
+    deallocation of a measured qubit (the fixed bits above it move down) and emulate_math.  Seeded
+    outcomes and the state after every step are compared with the oracle (simulator.hpp:145-186)."""
+    n, seed = 12, 91
+    print(f"known-zero subspaces {precision}", flush=True)
+    o = OracleSimulator(seed)
+    h = Simulator(seed, precision=precision)
+    tol = AMP_TOL if precision == "c128" else 1e-6
+    cmds = circuits.random_layers(n, 4, 5, seed, measure=False)
+    for sim in (o, h):
+        commands.execute(sim, cmds, False)
+
+    def same(tag):
+        mo, vo = o.cheat_numpy()
+        mh, vh = h.cheat_numpy()
+        assert mo == mh, tag
+        if precision == "c128":
+            assert np.max(np.abs(vo - vh)) <= tol, tag
+        else:
+            assert infidelity(vo, vh) <= 1e-6, tag
+
+    def both(fn, tag):
+        ro, rh = fn(o), fn(h)
+        assert ro == rh, (tag, ro, rh)
+        same(tag)
+        return ro
+
+    for q in (0, 1, 2, 7):
+        both(lambda s, q=q: s.measure_qubits([q]), f"measure {q}")
+    both(lambda s: s.measure_qubits([3, 9, 5]), "measure 3 9 5")
+    for sim in (o, h):
+        sim.apply_controlled_gate(G.Rz(0.7), [1], [])      # diagonal: 1 stays fixed
+        sim.apply_controlled_gate(G.H, [2], [])            # frees 2
+        sim.apply_controlled_gate(G.X, [4], [7])           # control on a measured qubit
+        sim.run()
+    same("gates")
+    both(lambda s: s.measure_qubits([2]), "measure 2 again")
+    both(lambda s: s.measure_qubits([1]), "measure 1 again")
+    v10 = bool((int(np.flatnonzero(np.abs(o.cheat_numpy()[1]) > 1e-9)[0]) >> 10) & 1)
+    for sim in (o, h):
+        sim.collapse_wavefunction([10], [v10])
+    same("collapse 10")
+    for sim in (o, h):
+        sim.allocate_qubit(n)
+        sim.apply_controlled_gate(G.H, [6], [])
+        sim.run()
+    both(lambda s: s.measure_qubits([n, 6]), "measure new qubit and 6")
+    for sim in (o, h):
+        sim.deallocate_qubit(1)
+    same("deallocate 1")
+    both(lambda s: s.measure_qubits([8]), "measure 8")
+    for sim in (o, h):
+        sim.emulate_math(lambda x: [(x[0] + 3) % 8], [[4, 5, 11]], [])
+        sim.apply_controlled_gate(G.H, [11], [])
+        sim.run()
+    same("math")
+    both(lambda s: s.measure_qubits([q for q in range(n + 1) if q != 1]), "measure all")
+
+
 MATH_CASES = [
     ("add_quantum", lambda: G.add_quantum(4), [[0, 1, 2, 3], [4, 5, 6, 7]], []),
     ("add_quantum_carry", lambda: G.add_quantum(3), [[0, 1, 2], [3, 4, 5], [6]], [7]),
