@@ -212,6 +212,12 @@ int qs_set_tile_params(qs_simulator *sim, uint32_t tile_bits, uint32_t reg_bits,
 #define QS_SLOT_LOOKAHEAD 0 /* highest decayed lookahead score (default) */
 #define QS_SLOT_EARLIEST 1  /* positions the earliest unscheduled gates need, in order of need */
 int qs_set_slot_rule(qs_simulator *sim, int rule);
+/* Qubit relabelling at tile stores (no counterpart in the reference): each tile pass moves into
+   the low positions, which every tile holds, the qubits of its tile that the remaining gates need
+   first; the exchange costs nothing (the store writes the tile with the bits exchanged) and the
+   qubit map follows.  -1: default (on for QS_PLANNER_LEARNED, off for QS_PLANNER_TILE), 0: off,
+   1: on.  Results are the same up to rounding; only the pass count changes. */
+int qs_set_relabel(qs_simulator *sim, int mode);
 /* The slot rule of the last window's tile plan (the learned planner tries both). */
 int qs_last_plan_rule(qs_simulator *sim, int *rule);
 /* Record every window pass: QS_PASS_FEATURES features and its GPU time. */

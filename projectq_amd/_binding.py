@@ -112,6 +112,7 @@ _SIGNATURES = {
     "qs_compare_states": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, _f64p, ctypes.POINTER(ctypes.c_uint64)]),
     "qs_set_tile_params": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32]),
     "qs_set_slot_rule": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
+    "qs_set_relabel": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     "qs_last_plan_rule": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int)]),
     "qs_set_recording": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     "qs_take_records": (
@@ -599,6 +600,13 @@ class CAbiSimulator:
     def set_slot_rule(self, rule):
         """Slot rule of the heuristic tile planner: "lookahead" (default) or "earliest"."""
         self._check(self._lib.qs_set_slot_rule(self._handle, self.SLOT_RULES.index(rule)))
+
+    RELABEL_MODES = {"default": -1, "off": 0, "on": 1}
+
+    def set_relabel(self, mode):
+        """Qubit relabelling at tile stores: "default" (on for the learned planner, off for the
+        heuristic tile planner), "off" or "on"."""
+        self._check(self._lib.qs_set_relabel(self._handle, self.RELABEL_MODES[mode]))
 
     def set_recording(self, enabled=True):
         self._check(self._lib.qs_set_recording(self._handle, 1 if enabled else 0))

@@ -359,6 +359,9 @@ API int qs_set_tile_params(qs_simulator *s, uint32_t t, uint32_t rb, uint32_t lo
 API int qs_set_slot_rule(qs_simulator *s, int rule) {
   return guarded(s, [&] { s->engine.set_slot_rule(rule); });
 }
+API int qs_set_relabel(qs_simulator *s, int mode) {
+  return guarded(s, [&] { s->engine.set_relabel(mode); });
+}
 API int qs_last_plan_rule(qs_simulator *s, int *rule) {
   return guarded(s, [&] { *rule = s->engine.last_plan().rule; });
 }

@@ -82,6 +82,11 @@ Engine::Engine(uint32_t seed, int precision, int fusion_mode, int planner, unsig
     else if (!std::strcmp(env, "1")) mfma_mode_ = 1;
     else throw ValueError("PROJECTQ_AMD_MFMA must be 0, 1 or all");
   }
+  if (const char *env = std::getenv("PROJECTQ_AMD_RELABEL")) {
+    if (!std::strcmp(env, "0")) relabel_mode_ = 0;
+    else if (!std::strcmp(env, "1")) relabel_mode_ = 1;
+    else throw ValueError("PROJECTQ_AMD_RELABEL must be 0 or 1");
+  }
   if (const char *env = std::getenv("PROJECTQ_AMD_TILE")) {
     unsigned t = 0, rb = 0;
     if (std::sscanf(env, "%u:%u", &t, &rb) != 2 || t < 6 || t > (c64_ ? 13u : 12u) || (rb != 4 && rb != 5))
@@ -220,6 +225,7 @@ void Engine::allocate_qubit(unsigned id) {
   QS_HIP(hipMemsetAsync((char *)psi_ + amps() * amp_bytes(), 0, amps() * amp_bytes(), stream_));
   pointer_out_ = false;
   subspace_fix(uint64_t(1) << n_, 0);  // the new qubit is |0>: the upper half is zero
+  ref_pos_[id] = n_;
   pos_[id] = n_++;
 }
 
@@ -241,7 +247,7 @@ bool Engine::is_classical(unsigned id, double tol) {
 // simulator.hpp:76-91: the first pair (bit clear, bit set) in index order holding an entry
 // above tol decides; the bit-clear entry is looked at first.
 bool Engine::get_classical_value(unsigned id, double tol) {
-  run();
+  canonicalize();  // the first pair in the reference's index order decides
   check_known({id}, "get_classical_value");
   *h_key_ = ~0ull;
   QS_HIP(hipMemcpyAsync(d_key_, h_key_, sizeof(unsigned long long), hipMemcpyHostToDevice, stream_));
@@ -258,7 +264,7 @@ bool Engine::get_classical_value(unsigned id, double tol) {
 
 // simulator.hpp:194-202 with collapse_vector(shrink) :110-143
 void Engine::deallocate_qubit(unsigned id) {
-  run();
+  canonicalize();
   check_known({id}, "deallocate_qubit");
   if (!is_classical(id, 1e-12))
     throw RuntimeError(
@@ -282,6 +288,10 @@ void Engine::deallocate_qubit(unsigned id) {
   for (auto &kv : pos_)
     if (kv.second > p) kv.second--;
   pos_.erase(id);
+  const unsigned rp = ref_pos_.at(id);
+  for (auto &kv : ref_pos_)
+    if (kv.second > rp) kv.second--;
+  ref_pos_.erase(id);
   n_--;
   pointer_out_ = false;
   const uint64_t low = (uint64_t(1) << p) - 1;
@@ -295,7 +305,7 @@ void Engine::deallocate_qubit(unsigned id) {
 // replaced by chunk sums (one read pass), a host prefix over the chunks and an in-chunk scan,
 // which lands on the same index unless the draw is within rounding of a boundary.
 std::vector<uint8_t> Engine::measure_qubits(const std::vector<unsigned> &ids) {
-  run();
+  canonicalize();  // the pick scans the reference's index order
   check_known(ids, "measure_qubits");
   const double rnd = uni_(rng_);
   const uint64_t N = amps();
@@ -514,10 +524,10 @@ unsigned Engine::tile_bits(unsigned tmax) const {
 }
 
 std::vector<TileProgram> Engine::plan_tiles(const std::vector<PGate> &pg, unsigned tmax, unsigned rb,
-                                            unsigned lmin, int rule, unsigned *t_used) const {
+                                            unsigned lmin, int rule, bool relabel, unsigned *t_used) const {
   const unsigned t = tile_bits(tmax);
   *t_used = t;
-  std::vector<PPass> passes = schedule_tiles(pg, TileShape{n_, t, lmin}, slot_chooser(rule));
+  std::vector<PPass> passes = schedule_tiles(pg, TileShape{n_, t, lmin}, slot_chooser(rule), 256, 16.0, relabel);
   std::vector<TileProgram> progs(passes.size());
   for (size_t i = 0; i < passes.size(); ++i) build_tile_pass(pg, passes[i], t, rb, c64_, progs[i]);
   return progs;
@@ -546,9 +556,88 @@ void Engine::flush_window() {
   if (info.kind == 0) {
     for (const Block &b : blocks) exec_block(b);
   } else {
-    for (const TileProgram &p : progs) exec_tile_program(p, rb);
+    for (const TileProgram &p : progs) {
+      exec_tile_program(p, rb);
+      apply_relabel(p.swaps);
+    }
   }
   after_passes();
+}
+
+// Restores the reference's layout (ref_pos_) after relabelling passes moved qubits.  One gather
+// pass into the scratch buffer; where that buffer cannot be allocated, in place with one exchange
+// of two index bits per misplaced qubit (at most n - 1, each touching half the state).
+void Engine::canonicalize() {
+  run();
+  if (pos_ == ref_pos_) return;
+  void *dst = buf_[0];
+  if (!dst && hipMalloc(&dst, cap_ * amp_bytes()) != hipSuccess) {
+    (void)hipGetLastError();  // clear the failed allocation's error state
+    dst = nullptr;
+  }
+  if (dst) {
+    buf_[0] = dst;
+    BitPerm perm{};
+    perm.count = n_;
+    uint64_t zm = 0, zv = 0;
+    for (const auto &kv : pos_) {
+      const unsigned from = kv.second, to = ref_pos_.at(kv.first);
+      perm.from[to] = (uint8_t)from;
+      zm |= ((zmask_ >> from) & 1u) << to;
+      zv |= ((zval_ >> from) & 1u) << to;
+    }
+    dispatch([&](auto tag) {
+      using R = decltype(tag);
+      dev::gather_bits<R>((const amp_t<R> *)psi_, (amp_t<R> *)dst, amps(), perm, stream_);
+    });
+    std::swap(psi_, buf_[0]);
+    sync();
+    pos_ = ref_pos_;
+    zmask_ = zm;
+    zval_ = zv;
+    return;
+  }
+  for (unsigned p = 0; p < n_; ++p) {
+    // the qubit that belongs at position p
+    unsigned want = 0;
+    bool found = false;
+    for (const auto &kv : ref_pos_)
+      if (kv.second == p) {
+        want = kv.first;
+        found = true;
+        break;
+      }
+    if (!found) throw RuntimeError("qubit layout: no qubit for a position");
+    const unsigned q = pos_.at(want);
+    if (q == p) continue;
+    dispatch([&](auto tag) {
+      using R = decltype(tag);
+      dev::swap_bits<R>((amp_t<R> *)psi_, amps(), (int)p, (int)q, stream_);
+    });
+    apply_relabel({{p, q}});
+  }
+  sync();
+}
+
+// A pass that relabels exchanged the qubits at these positions when it stored its tiles: the
+// qubit map and the known-zero bits follow.
+void Engine::apply_relabel(const std::vector<std::pair<uint32_t, uint32_t>> &swaps) {
+  if (swaps.empty()) return;
+  std::vector<int64_t> id_at(n_, -1);
+  for (const auto &kv : pos_) id_at[kv.second] = kv.first;
+  for (const auto &sw : swaps) {
+    const uint32_t a = sw.first, b = sw.second;
+    if (a >= n_ || b >= n_ || id_at[a] < 0 || id_at[b] < 0) throw RuntimeError("relabelling outside the state");
+    pos_[(unsigned)id_at[a]] = b;
+    pos_[(unsigned)id_at[b]] = a;
+    std::swap(id_at[a], id_at[b]);
+    auto swap_bits = [&](uint64_t &m) {
+      const uint64_t x = ((m >> a) ^ (m >> b)) & 1u;
+      m ^= (x << a) | (x << b);
+    };
+    swap_bits(zmask_);
+    swap_bits(zval_);
+  }
 }
 
 void Engine::plan_window(const std::vector<Gate> &gates, std::vector<Block> *blocks_out,
@@ -564,7 +653,7 @@ void Engine::plan_window(const std::vector<Gate> &gates, std::vector<Block> *blo
   const std::vector<PGate> pg = resolve(gates);
   if (planner_kind_ == 2) {
     unsigned t;
-    *progs_out = plan_tiles(pg, tile_tmax_, tile_rb_, tile_lmin_, slot_rule_, &t);
+    *progs_out = plan_tiles(pg, tile_tmax_, tile_rb_, tile_lmin_, slot_rule_, relabel_mode_ == 1, &t);
     *rb_out = tile_rb_;
     *info = PlanInfo{1, tile_tmax_, tile_rb_, tile_lmin_, (unsigned)progs_out->size(), 1, slot_rule_, 0.0};
     return;
@@ -588,7 +677,7 @@ void Engine::plan_window(const std::vector<Gate> &gates, std::vector<Block> *blo
     for (size_t c = 0; c < cands.size(); ++c)
       work.push_back(std::async(std::launch::async, [&, c] {
         unsigned t;
-        plans[c] = plan_tiles(pg, cands[c].tmax, cands[c].rb, cands[c].lmin, cands[c].rule, &t);
+        plans[c] = plan_tiles(pg, cands[c].tmax, cands[c].rb, cands[c].lmin, cands[c].rule, relabel_mode_ != 0, &t);
       }));
     for (auto &w : work) w.get();
   }
@@ -761,6 +850,11 @@ void Engine::set_tile_params(unsigned tmax, unsigned rb, unsigned lmin) {
   tile_lmin_ = lmin;
 }
 
+void Engine::set_relabel(int mode) {
+  if (mode < -1 || mode > 1) throw ValueError("relabel mode must be -1 (default), 0 (off) or 1 (on)");
+  relabel_mode_ = mode;
+}
+
 void Engine::set_slot_rule(int rule) {
   if (rule < 0 || rule >= kSlotRules) throw ValueError("unknown tile slot rule");
   run();
@@ -841,6 +935,8 @@ void Engine::reset_stats() {
 }
 
 cplx Engine::overlap(Engine &other) {
+  canonicalize();
+  other.canonicalize();
   run();
   other.run();
   other.sync();
@@ -852,6 +948,8 @@ cplx Engine::overlap(Engine &other) {
 }
 
 void Engine::compare(Engine &other, double *max_abs_diff, uint64_t *n_unequal) {
+  canonicalize();
+  other.canonicalize();
   run();
   other.run();
   other.sync();
@@ -923,7 +1021,7 @@ void Engine::math_apply(dev::MathArgs args, const int32_t *d_table, bool known_b
 void Engine::emulate_math_op(int opcode, const std::vector<int64_t> &params,
                              const std::vector<std::vector<unsigned>> &regs,
                              const std::vector<unsigned> &ctrl) {
-  run();
+  canonicalize();  // colliding amplitudes add in the reference's index order
   for (auto &r : regs) check_known(r, "emulate_math");
   check_known(ctrl, "emulate_math");
   dev::MathArgs a = math_args(regs, ctrl);
@@ -939,7 +1037,7 @@ void Engine::emulate_math_op(int opcode, const std::vector<int64_t> &params,
 void Engine::emulate_math_callback(const std::function<bool(int32_t *, size_t)> &fn,
                                    const std::vector<std::vector<unsigned>> &regs,
                                    const std::vector<unsigned> &ctrl) {
-  run();
+  canonicalize();  // colliding amplitudes add in the reference's index order
   for (auto &r : regs) check_known(r, "emulate_math");
   check_known(ctrl, "emulate_math");
   dev::MathArgs a = math_args(regs, ctrl);
@@ -1175,6 +1273,7 @@ void Engine::set_wavefunction(const double *wf, size_t n_amps, const std::vector
   if (n_amps != (uint64_t(1) << ordering.size()))
     throw ValueError("set_wavefunction(): the wavefunction must hold 2^len(ordering) amplitudes");
   for (size_t i = 0; i < ordering.size(); ++i) pos_[ordering[i]] = (unsigned)i;
+  ref_pos_ = pos_;
   if (c64_) {
     std::vector<float> f(2 * n_amps);
     for (size_t i = 0; i < 2 * n_amps; ++i) f[i] = (float)wf[i];
@@ -1217,14 +1316,14 @@ void Engine::collapse_wavefunction(const std::vector<unsigned> &ids, const std::
 
 // simulator.hpp:529-532
 void Engine::cheat_sizes(size_t *n_qubits, size_t *n_amps) {
-  run();
+  canonicalize();
   sync();
   *n_qubits = pos_.size();
   *n_amps = amps();
 }
 
 void Engine::cheat(uint32_t *ids, uint32_t *positions, double *state) {
-  run();
+  canonicalize();
   size_t k = 0;
   for (auto &kv : pos_) {
     ids[k] = kv.first;
@@ -1247,7 +1346,7 @@ void Engine::cheat(uint32_t *ids, uint32_t *positions, double *state) {
 // The state stays where it is: the pointer is valid until the next call on this simulator that
 // applies gates, allocates or frees qubits (those may move the state to the other buffer).
 void Engine::device_state(void **ptr, uint64_t *n_amps, int *precision) {
-  run();
+  canonicalize();
   sync();
   *ptr = psi_;
   *n_amps = amps();
@@ -1258,7 +1357,7 @@ void Engine::device_state(void **ptr, uint64_t *n_amps, int *precision) {
 }
 
 void Engine::qubit_map(uint32_t *ids, uint32_t *positions, size_t capacity, size_t *n_qubits) {
-  run();
+  canonicalize();
   size_t k = 0;
   for (auto &kv : pos_) {
     if (k < capacity) {
@@ -1306,7 +1405,7 @@ struct File {
 }  // namespace
 
 void Engine::save_state(const char *path) {
-  run();
+  canonicalize();
   sync();
   File file(path, "wb");
   StateHeader h{};
@@ -1389,6 +1488,7 @@ void Engine::load_state(const char *path) {
     for (unsigned id : by_pos) allocate_qubit(id);
   }
   pos_ = map;
+  ref_pos_ = map;
   const bool file_c64 = h.precision == 1;
   const size_t file_amp = file_c64 ? 8 : 16;
   const uint64_t total = h.n_amps * file_amp;
@@ -1448,6 +1548,7 @@ void Engine::set_wavefunction_device(const void *src, uint64_t n_amps, int src_p
     throw ValueError("set_wavefunction(): the wavefunction must hold 2^len(ordering) amplitudes");
   if (src_precision != 0 && src_precision != 1) throw ValueError("set_wavefunction_device: unknown precision");
   for (size_t i = 0; i < ordering.size(); ++i) pos_[ordering[i]] = (unsigned)i;
+  ref_pos_ = pos_;
   const bool src_c64 = src_precision == 1;
   if (src_c64 == c64_) {
     QS_HIP(hipMemcpyAsync(psi_, src, n_amps * amp_bytes(), hipMemcpyDeviceToDevice, stream_));

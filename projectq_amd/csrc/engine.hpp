@@ -74,6 +74,7 @@ class Engine {
   // planner controls (C-ABI extensions)
   void set_tile_params(unsigned tmax, unsigned rb, unsigned lmin);
   void set_slot_rule(int rule);
+  void set_relabel(int mode);
   void set_recording(bool on);
   size_t take_records(float *features, float *ms, size_t cap);
   void set_model(const std::vector<float> &w);
@@ -161,8 +162,12 @@ class Engine {
   std::vector<Block> greedy_blocks(const std::vector<Gate> &gates) const;
   std::vector<PGate> resolve(const std::vector<Gate> &gates) const;
   std::vector<TileProgram> plan_tiles(const std::vector<PGate> &pg, unsigned tmax, unsigned rb, unsigned lmin,
-                                      int rule, unsigned *t_used) const;
+                                      int rule, bool relabel, unsigned *t_used) const;
   int slot_rule_ = kSlotLookahead;  // rule of QS_PLANNER_TILE
+  // qubit relabelling at tile stores (schedule_tiles): -1 the default (on for the learned
+  // planner, off for QS_PLANNER_TILE), 0 off, 1 on; PROJECTQ_AMD_RELABEL=0|1 sets it
+  int relabel_mode_ = -1;
+  void apply_relabel(const std::vector<std::pair<uint32_t, uint32_t>> &swaps);
   void exec_tile_program(const TileProgram &prog, unsigned rb);
   PassFeatures block_features(const Block &b) const;
   void begin_pass(hipEvent_t *e0, hipEvent_t *e1);
@@ -194,6 +199,11 @@ class Engine {
   void *psi_ = nullptr;
   void *buf_[2] = {nullptr, nullptr};
   std::map<unsigned, unsigned> pos_;
+  // The reference's qubit layout (simulator.hpp: a new qubit takes the next position, a freed one
+  // closes the gap).  Relabelling tile stores move qubits away from it (pos_ is the layout of the
+  // state in HBM); every call that hands the layout out restores it first (canonicalize).
+  std::map<unsigned, unsigned> ref_pos_;
+  void canonicalize();
   GateQueue queue_;
   std::vector<Gate> window_;
   std::mt19937 rng_;

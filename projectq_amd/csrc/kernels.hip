@@ -548,6 +548,22 @@ __device__ __forceinline__ void tile_put16(uint32_t addr, const typename Vec16<R
     lds_store<typename Vec16<R>::type>(addr, v);
   }
 }
+// The same read when the stored tile relabels qubits: the pair (l, l + 1) of the stored tile comes
+// from computed-tile slots addr and addr ^ pair (pair = the slot bytes of the partner's bit).
+template <typename R>
+__device__ __forceinline__ typename Vec16<R>::type tile_get16p(uint32_t addr, uint32_t pair) {
+  if constexpr (sizeof(R) == 4) {
+    const re2_t<float> a = lds_load<re2_t<float>>(addr), b = lds_load<re2_t<float>>(addr ^ pair);
+    typename Vec16<R>::type v;
+    v.x = a.x;
+    v.y = a.y;
+    v.z = b.x;
+    v.w = b.y;
+    return v;
+  } else {
+    return lds_load<typename Vec16<R>::type>(addr);
+  }
+}
 template <typename R>
 __device__ __forceinline__ typename Vec16<R>::type tile_get16(uint32_t addr) {
   if constexpr (sizeof(R) == 4) {
@@ -595,6 +611,16 @@ __device__ __forceinline__ uint64_t tile_base(uint64_t tix, const TilePass &P) {
 
 // 16-byte accesses each thread keeps in flight while streaming a tile.
 constexpr int kStreamChunk = 8;
+
+// Tile-local index of the computed tile that index x of the stored tile takes (the relabelling of
+// P.sperm; x itself when there is none).  GF(2)-linear in x.
+__device__ __forceinline__ uint32_t store_source(uint32_t x, const TilePass &P) {
+  uint32_t y = 0;
+#pragma unroll
+  for (int b = 0; b < kTileMaxBits; ++b)
+    if (b < (int)P.t) y |= ((x >> b) & 1u) << P.sperm[b];
+  return y;
+}
 
 // The stages of a pass on one tile resident in LDS, a workgroup barrier after each.
 template <typename R, int RB, int TURN, int NT = kBlock>
@@ -655,13 +681,16 @@ __device__ __forceinline__ void tile_body(amp_t<R> *__restrict__ psi, uint64_t t
   }
   __syncthreads();
   run_stages<R, RB, 1>(tile, t, base, P.n_stages, stages, rops, tops, mats);
+  // the store reads the computed tile through the relabelling (linear: sources of disjoint bit
+  // sets combine by XOR)
+  const uint32_t src_thread = store_source(threadIdx.x * VN, P), pair = slot_bytes<R>(store_source(1, P));
   for (uint32_t l0 = threadIdx.x * VN; l0 < tsize; l0 += kStreamChunk * kStep) {
-    const uint32_t s0 = lds_address(tile) + slot_bytes<R>(l0);  // l0 shares no bit with j * kStep
+    const uint32_t s0 = lds_address(tile) + slot_bytes<R>(src_thread ^ store_source(l0 - threadIdx.x * VN, P));
     V v[kStreamChunk];
 #pragma unroll
     for (int j = 0; j < kStreamChunk; ++j) {
       const uint32_t l = l0 + j * kStep;
-      if (FULL || l < tsize) v[j] = tile_get16<R>(s0 ^ slot_bytes<R>(j * kStep));
+      if (FULL || l < tsize) v[j] = tile_get16p<R>(s0 ^ slot_bytes<R>(store_source(j * kStep, P)), pair);
     }
 #pragma unroll
     for (int j = 0; j < kStreamChunk; ++j) {
@@ -726,6 +755,13 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_
   const uint32_t segmask = (1u << L) - 1;
   const uint32_t l0 = threadIdx.x * VN;
   const uint32_t s0 = lds_address(tile) + slot_bytes<R>(l0);  // l0 shares no bit with j * kStep
+  // the store reads the computed tile through the relabelling of P.sperm (identity: s0, j * kStep)
+  const uint32_t ss0 = lds_address(tile) + slot_bytes<R>(store_source(l0, P));
+  const uint32_t spair = slot_bytes<R>(store_source(1, P));
+  // j * kStep for j < 8 spans three bits: its source is the XOR of theirs
+  const uint32_t sbit0 = slot_bytes<R>(store_source(kStep, P)), sbit1 = slot_bytes<R>(store_source(2 * kStep, P)),
+                 sbit2 = slot_bytes<R>(store_source(4 * kStep, P));
+  static_assert(kStreamChunk == 8, "three step bits");
   // Each turn of the loop runs the stages of the resident tile and stores it, then moves the
   // registers holding the next tile (loaded during those stages) into LDS and starts loading the
   // tile after.  At that wait only the older loads must be complete, not the stores just issued.
@@ -761,7 +797,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_
     for (int j = 0; j < kStreamChunk; ++j) {
       const uint32_t l = ls + j * kStep;
       stream_store(reinterpret_cast<V *>(psi + (base | seg[l >> L] | (l & segmask))),
-                   tile_get16<R>(s0 ^ slot_bytes<R>(j * kStep)));
+                   tile_get16p<R>(ss0 ^ ((j & 1) ? sbit0 : 0u) ^ ((j & 2) ? sbit1 : 0u) ^ ((j & 4) ? sbit2 : 0u), spair));
     }
     __syncthreads();
     if (next >= n_tiles) break;  // uniform over the workgroup
@@ -956,6 +992,32 @@ __global__ __launch_bounds__(kBlock) void k_drop_bit(const amp_t<R> *__restrict_
   const uint64_t set = uint64_t(value ? 1 : 0) << p;
   for (uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; j < n_out; j += stride)
     dst[j] = src[insert_one_zero(j, p) | set];
+}
+
+// A qubit layout change, one exchange of two index bits at a time: the amplitudes whose bits a and
+// b differ trade places (half the state read and written; a one-off when the layout is handed out).
+template <typename R>
+__global__ __launch_bounds__(kBlock) void k_swap_bits(amp_t<R> *__restrict__ psi, uint64_t n_pairs, int lo, int hi) {
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t c = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; c < n_pairs; c += stride) {
+    const uint64_t i = insert_one_zero(insert_one_zero(c, lo), hi) | (uint64_t(1) << lo);
+    const uint64_t j = i ^ (uint64_t(1) << lo) ^ (uint64_t(1) << hi);
+    const amp_t<R> x = psi[i], y = psi[j];
+    psi[i] = y;
+    psi[j] = x;
+  }
+}
+
+// The same layout change in one pass into another buffer: writes in order, reads gathered.
+template <typename R>
+__global__ __launch_bounds__(kBlock) void k_gather_bits(const amp_t<R> *__restrict__ src, amp_t<R> *__restrict__ dst,
+                                                        uint64_t n, BitPerm perm) {
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; j < n; j += stride) {
+    uint64_t i = 0;
+    for (uint32_t b = 0; b < perm.count; ++b) i |= ((j >> b) & 1u) << perm.from[b];
+    dst[j] = src[i];
+  }
 }
 
 // ------------------------------------------------------------------------------ Pauli strings
@@ -1495,6 +1557,20 @@ void drop_bit(const amp_t<R> *src, amp_t<R> *dst, uint64_t n_out, int p, int val
 }
 
 template <typename R>
+void swap_bits(amp_t<R> *psi, uint64_t n_amps, int a, int b, hipStream_t s) {
+  if (a == b || n_amps < 4) return;
+  const int lo = a < b ? a : b, hi = a < b ? b : a;
+  hipLaunchKernelGGL(k_swap_bits<R>, dim3(grid_for(n_amps / 4)), dim3(kBlock), 0, s, psi, n_amps / 4, lo, hi);
+  QS_HIP(hipGetLastError());
+}
+
+template <typename R>
+void gather_bits(const amp_t<R> *src, amp_t<R> *dst, uint64_t n_amps, const BitPerm &perm, hipStream_t s) {
+  hipLaunchKernelGGL(k_gather_bits<R>, dim3(grid_for(n_amps)), dim3(kBlock), 0, s, src, dst, n_amps, perm);
+  QS_HIP(hipGetLastError());
+}
+
+template <typename R>
 void pauli_expect(const amp_t<R> *psi, uint64_t n, uint64_t x, uint64_t z, double *partials,
                   hipStream_t s) {
   hipLaunchKernelGGL(k_pauli_expect<R>, dim3(kReduceGrid), dim3(kBlock), 0, s, psi, n, x, z,
@@ -1623,6 +1699,8 @@ template void convert_amps<float, double>(const amp_t<float> *, amp_t<double> *,
   template void first_pair<R>(const amp_t<R> *, uint64_t, int, double, unsigned long long *,      \
                               hipStream_t);                                                       \
   template void drop_bit<R>(const amp_t<R> *, amp_t<R> *, uint64_t, int, int, hipStream_t);       \
+  template void swap_bits<R>(amp_t<R> *, uint64_t, int, int, hipStream_t);                          \
+  template void gather_bits<R>(const amp_t<R> *, amp_t<R> *, uint64_t, const BitPerm &, hipStream_t);  \
   template void pauli_expect<R>(const amp_t<R> *, uint64_t, uint64_t, uint64_t, double *,         \
                                 hipStream_t);                                                     \
   template void pauli_energy<R>(const amp_t<R> *, uint64_t, const PauliTerm *, int, double *,      \

@@ -16,6 +16,13 @@ struct BitInsert {
   uint8_t pos[kMaxBits];
 };
 
+// A permutation of index bits: bit b of a destination index is bit from[b] of the source index.
+struct BitPerm {
+  uint32_t count;
+  uint8_t from[kMaxBits];
+};
+
+
 namespace dev {
 
 // ---- tile passes: one HBM read + write of the state, many gates applied in LDS ----------------
@@ -75,6 +82,10 @@ struct TilePass {
   uint32_t n_stages;
   uint32_t hpos[kTileMaxHigh];  // global positions of the high tile bits, ascending
   BitInsert tiles;              // all tile positions (global, ascending): enumerates tile bases
+  // Qubit relabelling at the store: tile-local bit b of the stored tile takes bit sperm[b] of the
+  // tile as computed (sperm[b] = b: no relabelling).  A product of bit exchanges, so the stored
+  // amplitude at tile-local index l is the computed one at sum_b bit_b(l) << sperm[b].
+  uint32_t sperm[kTileMaxBits];
 };
 
 // One pass of one circuit inside a batched launch (k_batch_tile_pass).  Offsets index the

@@ -81,7 +81,7 @@ static bool diagonal_gate(const PGate &g) {
 }
 
 std::vector<PPass> schedule_tiles(const std::vector<PGate> &gates, const TileShape &shape,
-                                  const SlotChooser &choose, unsigned lookahead, double decay) {
+                                  const SlotChooser &choose, unsigned lookahead, double decay, bool relabel) {
   const int G = (int)gates.size();
   const unsigned n = shape.n, t = std::min(shape.t, shape.n);
   const unsigned L0 = std::min(shape.Lmin, t);
@@ -113,12 +113,16 @@ std::vector<PPass> schedule_tiles(const std::vector<PGate> &gates, const TileSha
   }
   std::vector<char> done(G, 0);
   std::vector<PPass> passes;
+  // label (a position in `gates`) <-> position in the state; changed only by relabelling
+  std::vector<uint32_t> phys(std::max(n, 1u)), label(std::max(n, 1u));
+  for (uint32_t p = 0; p < phys.size(); ++p) phys[p] = label[p] = p;
+  auto pos_of = [&](uint32_t lab) { return lab < phys.size() ? phys[lab] : lab; };
   int first = 0;
   while (first < G) {
     // forced high positions: the targets of the first unscheduled gate (always ready)
     std::set<uint32_t> high;
-    for (uint32_t p : gates[first].tpos)
-      if (p >= L0) high.insert(p);
+    for (uint32_t lab : gates[first].tpos)
+      if (pos_of(lab) >= L0) high.insert(pos_of(lab));
     const unsigned slots = t - L0;
     // lookahead statistics over the next unscheduled gates
     std::vector<SlotCandidate> cand;
@@ -127,7 +131,8 @@ std::vector<PPass> schedule_tiles(const std::vector<PGate> &gates, const TileSha
       unsigned rank = 0;
       for (int i = first; i < G && rank < lookahead; ++i) {
         if (done[i]) continue;
-        for (uint32_t p : gates[i].tpos) {
+        for (uint32_t lab : gates[i].tpos) {
+          const uint32_t p = pos_of(lab);
           if (p < L0) continue;
           score[p] += 1.0 / (1.0 + rank / decay);
           cnt[p] += 1.0;
@@ -161,7 +166,9 @@ std::vector<PPass> schedule_tiles(const std::vector<PGate> &gates, const TileSha
     size_t upload = 0;
     for (int i = first; i < sweep_end; ++i) {
       if (done[i]) continue;
-      if ((gates[i].tmask() & ~tmask) != 0) continue;
+      uint64_t tm = 0;
+      for (uint32_t lab : gates[i].tpos) tm |= uint64_t(1) << pos_of(lab);
+      if ((tm & ~tmask) != 0) continue;
       bool ready = true;
       for (int p : preds[i])
         if (!done[p]) {
@@ -175,8 +182,37 @@ std::vector<PPass> schedule_tiles(const std::vector<PGate> &gates, const TileSha
       done[i] = 1;
       pass.gates.push_back(i);
     }
-    passes.push_back(std::move(pass));
+    if (relabel) pass.phys = phys;
     while (first < G && done[first]) ++first;
+    if (relabel && first < G && L0 > 0) {
+      // the low positions take the tile's qubits that the remaining gates need first
+      std::vector<double> need(phys.size(), 1e300);
+      unsigned rank = 0;
+      for (int i = first; i < G && rank < lookahead; ++i) {
+        if (done[i]) continue;
+        for (uint32_t lab : gates[i].tpos) need[pos_of(lab)] = std::min(need[pos_of(lab)], (double)rank);
+        ++rank;
+      }
+      std::vector<uint32_t> cands;
+      for (uint32_t p = 0; p < L0; ++p) cands.push_back(p);
+      for (uint32_t p : high) cands.push_back(p);
+      std::stable_sort(cands.begin(), cands.end(), [&](uint32_t a, uint32_t b) { return need[a] < need[b]; });
+      const std::set<uint32_t> want(cands.begin(), cands.begin() + L0);
+      std::vector<uint32_t> incoming;
+      for (uint32_t p : want)
+        if (p >= L0) incoming.push_back(p);
+      size_t k = 0;
+      for (uint32_t lo = 0; lo < L0 && k < incoming.size(); ++lo) {
+        if (want.count(lo)) continue;
+        const uint32_t hi = incoming[k++];
+        pass.swaps.push_back({lo, hi});
+        const uint32_t a = label[lo], b = label[hi];
+        std::swap(label[lo], label[hi]);
+        phys[a] = hi;
+        phys[b] = lo;
+      }
+    }
+    passes.push_back(std::move(pass));
   }
   return passes;
 }
@@ -512,6 +548,16 @@ void build_tile_pass(const std::vector<PGate> &gates, const PPass &ps, unsigned 
   P.tiles.count = (uint32_t)tilepos.size();
   for (size_t i = 0; i < tilepos.size(); ++i) P.tiles.pos[i] = (uint8_t)tilepos[i];
 
+  // the store's relabelling: exchanging the qubits at positions a and b exchanges tile bits
+  for (uint32_t b = 0; b < (uint32_t)dev::kTileMaxBits; ++b) P.sperm[b] = b;
+  for (const auto &sw : ps.swaps) {
+    if (sw.first >= 64 || sw.second >= 64 || loc[sw.first] < 0 || loc[sw.second] < 0 || sw.first == sw.second)
+      throw std::runtime_error("tile planner: a relabelled position lies outside its tile");
+    std::swap(P.sperm[loc[sw.first]], P.sperm[loc[sw.second]]);
+  }
+  prog.swaps = ps.swaps;
+  auto at = [&](uint32_t lab) -> uint32_t { return ps.phys.empty() ? lab : (lab < ps.phys.size() ? ps.phys[lab] : 64u); };
+
   std::vector<LocalGate> lgs;
   std::vector<char> diag;
   for (int gi : ps.gates) {
@@ -521,13 +567,16 @@ void build_tile_pass(const std::vector<PGate> &gates, const PPass &ps, unsigned 
     if (lg.k < 1 || lg.k > 5) throw std::runtime_error("tile pass: gate width outside 1..5");
     uint32_t need = 0;
     for (uint32_t l = 0; l < lg.k; ++l) {
-      if (g.tpos[l] >= 64 || loc[g.tpos[l]] < 0) throw std::runtime_error("tile planner placed a gate outside its tile");
-      lg.tb[l] = (uint32_t)loc[g.tpos[l]];
+      const uint32_t p = at(g.tpos[l]);
+      if (p >= 64 || loc[p] < 0) throw std::runtime_error("tile planner placed a gate outside its tile");
+      lg.tb[l] = (uint32_t)loc[p];
       need |= 1u << lg.tb[l];
     }
     if (popcount32(need) != lg.k) throw std::runtime_error("tile pass: repeated target");
-    for (uint32_t p = 0; p < 64; ++p) {
-      if (!((g.cmask >> p) & 1)) continue;
+    for (uint32_t lab = 0; lab < 64; ++lab) {
+      if (!((g.cmask >> lab) & 1)) continue;
+      const uint32_t p = at(lab);
+      if (p >= 64) throw std::runtime_error("tile pass: control outside the state");
       if (loc[p] >= 0) lg.lc |= 1u << loc[p];
       else lg.gc |= uint64_t(1) << p;
     }

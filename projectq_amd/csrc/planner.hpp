@@ -30,10 +30,15 @@ struct PGate {
 };
 
 // One tile pass: low tile bits 0..L-1 plus the `high` positions; gates in execution order.
+// With qubit relabelling, the gates' positions are labels: `phys` maps each label to its position
+// in the state during this pass (empty: the identity), and `swaps` lists the pairs of positions
+// (both in the tile) whose qubits the pass exchanges when it stores its tiles.
 struct PPass {
   uint32_t L = 0;
   std::vector<uint32_t> high;
   std::vector<int> gates;
+  std::vector<uint32_t> phys;
+  std::vector<std::pair<uint32_t, uint32_t>> swaps;
 };
 
 struct TileShape {
@@ -72,9 +77,13 @@ inline size_t gate_upload_bytes(const PGate &g) {
 // Tile scheduling: repeatedly pick a tile (forced: the targets of the first unscheduled gate;
 // the other high slots from `choose`, default = highest lookahead score) and take, in queue
 // order, every unscheduled gate whose predecessors are scheduled and whose targets are in it.
+// With `relabel`, each pass then moves into the low positions (present in every tile) the qubits
+// of its tile that the remaining gates need first, exchanging them with low qubits needed later;
+// the exchange is a free relabelling at the pass's store (PPass::swaps), and later passes see
+// the gates at their new positions (PPass::phys).
 std::vector<PPass> schedule_tiles(const std::vector<PGate> &gates, const TileShape &shape,
                                   const SlotChooser &choose = nullptr, unsigned lookahead = 256,
-                                  double decay = 16.0);
+                                  double decay = 16.0, bool relabel = false);
 
 // Device description of one tile pass.  Gates are split into stages: runs of 1- and 2-qubit
 // gates whose targets fit `rb` register bits become one register stage (RegOps, specialised
@@ -86,6 +95,7 @@ struct TileProgram {
   std::vector<dev::RegOp> rops;
   std::vector<dev::TileOp> tops;
   std::vector<std::complex<double>> mats;
+  std::vector<std::pair<uint32_t, uint32_t>> swaps;  // positions whose qubits the store exchanges
 };
 void build_tile_pass(const std::vector<PGate> &gates, const PPass &pass, unsigned t, unsigned rb, bool c64,
                      TileProgram &prog);

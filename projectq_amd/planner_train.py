@@ -3,9 +3,10 @@
     python -m projectq_amd.planner_train --circuits 2000 --out projectq_amd/planner_mlp.npz
 
 1. Generate seeded random circuits (24..30 qubits: mixed 1/2/3-qubit gates, random layers with
-   CNOTs, or a QFT followed by layers of rotations and multi-controlled Rz), run each once in window mode with a random planner configuration (tile passes of a
-   random shape under a random slot rule, or greedy max-5 blocks), recording every pass: its
-   features and its GPU time.
+   CNOTs, or a QFT followed by layers of rotations and multi-controlled Rz), run each once in
+   window mode with a random planner configuration (tile passes of a random shape under a random
+   slot rule, with or without qubit relabelling, or greedy max-5 blocks), recording every pass:
+   its features and its GPU time.
 2. Train the 24 -> 64 -> 64 -> 1 MLP on log(time per 2^20 amplitudes) with the HIP training
    kernels (qs_mlp_train), holding out 10% of the circuits for validation.
 3. Save the weights with the training statistics.
@@ -62,6 +63,7 @@ def generate(n_circuits, qmin, qmax, seed):
         if not greedy:
             sim.set_tile_params(*TILE_SHAPES[prec][int(rng.integers(len(TILE_SHAPES[prec])))])
             sim.set_slot_rule(Simulator.SLOT_RULES[int(rng.integers(len(Simulator.SLOT_RULES)))])
+            sim.set_relabel("on" if rng.random() < 0.5 else "off")
         for q in range(n):
             sim.allocate_qubit(q)
         sim.run()

@@ -100,6 +100,13 @@ void replay(cplx *psi, unsigned n, const qs::TileProgram &prog, unsigned rb, boo
     seg[s] = o;
   }
   const uint64_t n_tiles = N >> t;
+  {
+    uint32_t seen = 0;
+    for (uint32_t b = 0; b < t; ++b) {
+      check(P.sperm[b] < t && !((seen >> P.sperm[b]) & 1), "store relabelling is a permutation of the tile bits");
+      seen |= 1u << P.sperm[b];
+    }
+  }
   std::vector<cplx> tile(tsize);
   auto lds = [&](uint32_t x) -> cplx & {
     check(x < tsize, "LDS index");
@@ -185,7 +192,12 @@ void replay(cplx *psi, unsigned n, const qs::TileProgram &prog, unsigned rb, boo
         }
       }
     }
-    for (uint32_t l = 0; l < tsize; ++l) psi[base | seg[l >> L] | (l & segmask)] = lds(l);
+    // the store: stored index l takes the computed amplitude at the relabelled index
+    for (uint32_t l = 0; l < tsize; ++l) {
+      uint32_t src = 0;
+      for (uint32_t b = 0; b < t; ++b) src |= ((l >> b) & 1u) << P.sperm[b];
+      psi[base | seg[l >> L] | (l & segmask)] = lds(src);
+    }
   }
 }
 
@@ -250,7 +262,46 @@ int emu_plan_stats(unsigned n, unsigned t, unsigned lmin, unsigned rb, int c64, 
 }
 
 // gates: k[i] targets at tpos[off..off+k), controls cmask[i], matrix at mats[moff..] (complex
-// interleaved, 4^k entries).  state: 2^n complex interleaved, updated in place.
+// interleaved, 4^k entries).  state: 2^n complex interleaved, updated in place.  With `relabel`
+// the passes relabel qubits at their stores (rule: slot rule of the scheduler); final_pos[label]
+// receives the position each label ends at (the state is left in that layout).
+int emu_run_relabel(unsigned n, unsigned t, unsigned lmin, unsigned rb, int c64, int relabel, int rule, int n_gates,
+                    const uint32_t *k, const uint32_t *tpos, const uint64_t *cmask, const double *mats, double *state,
+                    int *n_passes, uint32_t *final_pos, char *err, int err_len) {
+  try {
+    auto gates = qs::merge_single_qubit_runs(read_gates(n_gates, k, tpos, cmask, mats));
+    auto passes = qs::schedule_tiles(gates, qs::TileShape{n, t, lmin}, qs::slot_chooser(rule), 256, 16.0, relabel != 0);
+    std::vector<uint32_t> label(n), phys(n);
+    for (uint32_t p = 0; p < n; ++p) label[p] = phys[p] = p;
+    size_t covered = 0;
+    for (auto &ps : passes) {
+      if (relabel) check(ps.phys == phys, "pass positions follow the relabelling");
+      qs::TileProgram prog;
+      qs::build_tile_pass(gates, ps, std::min(t, n), rb, c64 != 0, prog);
+      if (state) replay(reinterpret_cast<cplx *>(state), n, prog, rb, c64 != 0);
+      for (const auto &sw : prog.swaps) {
+        check(sw.first < n && sw.second < n, "relabelled positions");
+        const uint32_t a = label[sw.first], b = label[sw.second];
+        std::swap(label[sw.first], label[sw.second]);
+        phys[a] = sw.second;
+        phys[b] = sw.first;
+      }
+      covered += ps.gates.size();
+    }
+    check(covered == gates.size(), "every gate scheduled exactly once");
+    if (final_pos)
+      for (uint32_t p = 0; p < n; ++p) final_pos[p] = phys[p];
+    *n_passes = (int)passes.size();
+    return 0;
+  } catch (const Fail &f) {
+    std::strncpy(err, f.what.c_str(), err_len - 1);
+    return 1;
+  } catch (const std::exception &e) {
+    std::strncpy(err, e.what(), err_len - 1);
+    return 2;
+  }
+}
+
 int emu_run(unsigned n, unsigned t, unsigned lmin, unsigned rb, int c64, int n_gates, const uint32_t *k, const uint32_t *tpos,
             const uint64_t *cmask, const double *mats, double *state, int *n_passes, char *err, int err_len) {
   try {

@@ -198,6 +198,49 @@ def test_measurements_on_known_zero_subspaces(precision):
     both(lambda s: s.measure_qubits([q for q in range(n + 1) if q != 1]), "measure all")
 
 
+@pytest.mark.parametrize("planner", ["tile", "learned"])
+def test_relabelling_windows_keep_the_reference_layout(planner):
+    """Windows whose tile passes relabel qubits (the qubits needed next move into the low
+    positions at each store) leave the state in another layout; everything that exposes the layout
+    or depends on the reference's index order sees the reference's: cheat, seeded measurements,
+    emulate_math with colliding amplitudes, deallocation.  complex128, compared with the oracle
+    to 1e-12, with identical seeded outcomes."""
+    n, seed = 20, 93
+    print(f"relabelling windows planner={planner}", flush=True)
+    o = OracleSimulator(seed)
+    h = Simulator(seed, precision="c128", fusion="window", planner=planner)
+    h.set_tile_params(12, 4, 4)
+    h.set_relabel("on")
+    cmds = circuits.random_layers(n, 12, 10, seed, measure=False)
+    for sim in (o, h):
+        commands.execute(sim, cmds, True)
+        sim.run()
+    mo, vo = o.cheat_numpy()
+    mh, vh = h.cheat_numpy()
+    assert mo == mh
+    assert np.max(np.abs(vo - vh)) <= AMP_TOL
+    # more gates, then a non-injective map: collisions add in the reference's index order
+    more = circuits.random_layers(n, 4, 10, seed + 1, measure=False)[n:]
+    for sim in (o, h):
+        commands.execute(sim, more, True)
+        sim.run()
+        sim.emulate_math(lambda x: [x[0] // 2, x[1]], [[0, 5, 9], [12, 17]], [])
+    assert np.max(np.abs(o.cheat_numpy()[1] - h.cheat_numpy()[1])) <= AMP_TOL
+    for sim in (o, h):
+        commands.execute(sim, circuits.random_layers(n, 4, 10, seed + 2, measure=False)[n:], True)
+        sim.run()
+    outcomes_o = [o.measure_qubits([q])[0] for q in range(n)]
+    outcomes_h = [h.measure_qubits([q])[0] for q in range(n)]
+    assert outcomes_o == outcomes_h
+    for sim in (o, h):
+        for q in (3, 11, 0):
+            sim.deallocate_qubit(q)
+    mo, vo = o.cheat_numpy()
+    mh, vh = h.cheat_numpy()
+    assert mo == mh
+    assert np.max(np.abs(vo - vh)) <= AMP_TOL
+
+
 MATH_CASES = [
     ("add_quantum", lambda: G.add_quantum(4), [[0, 1, 2, 3], [4, 5, 6, 7]], []),
     ("add_quantum_carry", lambda: G.add_quantum(3), [[0, 1, 2], [3, 4, 5], [6]], [7]),
@@ -418,7 +461,7 @@ def test_tile_pass_2p11_tiles_25q_c128():
 
 def test_headline_shape_32q_round_trip():
     """The measured configuration at its full size (32 qubits, complex64, window fusion, learned
-    planner, so the pipelined tile pass on 2^20 tiles): three layers of the random-layer circuit
+    planner, so the pipelined tile pass on 2^20 or 2^19 tiles): three layers of the random-layer circuit
     followed by their inverse return the state to |0...0>.  A size-independent check of the
     kernel on the real shape, where no CPU reference fits."""
     from projectq_amd import Simulator
@@ -434,7 +477,7 @@ def test_headline_shape_32q_round_trip():
         sim.apply_controlled_gate(m, list(t), list(c))
     sim.run()
     plan = sim.last_plan()
-    assert plan["kind"] == "tile" and plan["tile_bits"] == 12
+    assert plan["kind"] == "tile" and plan["tile_bits"] in (12, 13), plan
     zero = [False] * n
     p0 = sim.get_probability(zero, list(range(n)))
     assert p0 >= 1 - 1e-4, p0
