@@ -218,6 +218,8 @@ int qs_set_slot_rule(qs_simulator *sim, int rule);
    qubit map follows.  -1: default (on for QS_PLANNER_LEARNED, off for QS_PLANNER_TILE), 0: off,
    1: on.  Results are the same up to rounding; only the pass count changes. */
 int qs_set_relabel(qs_simulator *sim, int mode);
+/* How many passes of the last window's tile plan relabelled qubits at their stores. */
+int qs_last_plan_relabels(qs_simulator *sim, uint32_t *passes);
 /* The slot rule of the last window's tile plan (the learned planner tries both). */
 int qs_last_plan_rule(qs_simulator *sim, int *rule);
 /* Record every window pass: QS_PASS_FEATURES features and its GPU time. */

@@ -937,6 +937,10 @@ API int qs_compare_states(qs_simulator *a, qs_simulator *b, double *max_abs_diff
 API int qs_set_tile_params(qs_simulator *, uint32_t, uint32_t, uint32_t) { return QS_OK; }
 API int qs_set_slot_rule(qs_simulator *, int) { return QS_OK; }
 API int qs_set_relabel(qs_simulator *, int) { return QS_OK; }
+API int qs_last_plan_relabels(qs_simulator *, uint32_t *passes) {
+  *passes = 0;
+  return QS_OK;
+}
 API int qs_last_plan_rule(qs_simulator *, int *rule) {
   *rule = 0;
   return QS_OK;

@@ -113,6 +113,7 @@ _SIGNATURES = {
     "qs_set_tile_params": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32]),
     "qs_set_slot_rule": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     "qs_set_relabel": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
+    "qs_last_plan_relabels": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_uint32)]),
     "qs_last_plan_rule": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int)]),
     "qs_set_recording": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     "qs_take_records": (
@@ -648,9 +649,12 @@ class CAbiSimulator:
         )
         rule = ctypes.c_int()
         self._check(self._lib.qs_last_plan_rule(self._handle, ctypes.byref(rule)))
+        relabels = ctypes.c_uint32()
+        self._check(self._lib.qs_last_plan_relabels(self._handle, ctypes.byref(relabels)))
         return {
             "kind": {0: "greedy_blocks", 1: "tile"}.get(kind.value, "none"),
             "rule": self.SLOT_RULES[rule.value] if kind.value == 1 else None,
+            "relabels": relabels.value,
             "tile_bits": t.value,
             "reg_bits": rb.value,
             "low_bits": low.value,

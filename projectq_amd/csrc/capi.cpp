@@ -362,6 +362,9 @@ API int qs_set_slot_rule(qs_simulator *s, int rule) {
 API int qs_set_relabel(qs_simulator *s, int mode) {
   return guarded(s, [&] { s->engine.set_relabel(mode); });
 }
+API int qs_last_plan_relabels(qs_simulator *s, uint32_t *passes) {
+  return guarded(s, [&] { *passes = s->engine.last_plan().relabels; });
+}
 API int qs_last_plan_rule(qs_simulator *s, int *rule) {
   return guarded(s, [&] { *rule = s->engine.last_plan().rule; });
 }

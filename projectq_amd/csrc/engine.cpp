@@ -551,6 +551,7 @@ void Engine::flush_window() {
   PlanInfo info;
   plan_window(window_, &blocks, &progs, &rb, &info);
   window_.clear();
+  for (const TileProgram &p : progs) info.relabels += p.swaps.empty() ? 0u : 1u;
   last_plan_ = info;
   before_unitary_passes();
   if (info.kind == 0) {

@@ -83,6 +83,7 @@ class Engine {
     unsigned tmax = 0, rb = 0, lmin = 0, passes = 0, candidates = 0;
     int rule = 0;   // tile slot rule (SlotRule)
     double predicted_ms = 0.0;
+    unsigned relabels = 0;  // tile passes that relabel qubits at their stores
   };
   const PlanInfo &last_plan() const { return last_plan_; }
 

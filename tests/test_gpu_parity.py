@@ -215,6 +215,8 @@ def test_relabelling_windows_keep_the_reference_layout(planner):
     for sim in (o, h):
         commands.execute(sim, cmds, True)
         sim.run()
+    if planner == "tile" or h.last_plan()["kind"] == "tile":
+        assert h.last_plan()["relabels"] > 0, h.last_plan()
     mo, vo = o.cheat_numpy()
     mh, vh = h.cheat_numpy()
     assert mo == mh

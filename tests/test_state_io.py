@@ -103,6 +103,43 @@ def test_state_file_round_trip_large_gpu(tmp_path):
 
 
 @pytest.mark.gpu
+def test_state_handed_out_after_relabelling_windows(tmp_path):
+    """Tile passes that relabel qubits leave the state in another layout; the checkpoint, the qubit
+    map and the device state are in the reference's layout (checked against the oracle), and a
+    checkpoint written after relabelling loads back into a simulator whose next window relabels
+    again."""
+    from projectq_amd import Simulator
+
+    n = 18
+    cmds = circuits.random_layers(n, 10, 9, 12, measure=False)
+    o = OracleSimulator(4)
+    commands.execute(o, cmds, gate_fusion=False)
+    h = Simulator(4, precision="c128", fusion="window", planner="tile", capacity_qubits=n)
+    h.set_tile_params(11, 4, 4)
+    h.set_relabel("on")
+    commands.execute(h, cmds, gate_fusion=True)
+    h.run()
+    assert h.last_plan()["relabels"] > 0, h.last_plan()
+    path = str(tmp_path / "relabelled.qsst")
+    h.save_state(path)
+    mapping, view = h.device_state()
+    assert view.n_amps == 2**n and not view.c64
+    assert mapping == {q: q for q in range(n)} == h.qubit_map()  # the reference's layout
+    want = o.cheat_numpy()[1]
+    assert np.max(np.abs(h.cheat_numpy()[1] - want)) <= 1e-12
+    g = Simulator(4, precision="c128", fusion="window", planner="tile", capacity_qubits=n)
+    g.set_tile_params(11, 4, 4)
+    g.set_relabel("on")
+    g.load_state(path)
+    assert np.max(np.abs(g.cheat_numpy()[1] - want)) <= 1e-12
+    more = circuits.random_layers(n, 3, 9, 13, measure=False)[n:]
+    commands.execute(o, more, gate_fusion=False)
+    commands.execute(g, more, gate_fusion=True)
+    g.run()
+    assert np.max(np.abs(g.cheat_numpy()[1] - o.cheat_numpy()[1])) <= 1e-12
+
+
+@pytest.mark.gpu
 def test_malformed_state_file_leaves_the_gpu_simulator_unchanged(tmp_path):
     """A truncated or padded checkpoint is rejected (ValueError) before the qubit map or the
     state change."""
