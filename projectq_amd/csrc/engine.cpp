@@ -565,8 +565,8 @@ void Engine::flush_window() {
   after_passes();
 }
 
-// Restores the reference's layout (ref_pos_) after relabelling passes moved qubits.  One gather
-// pass into the scratch buffer; where that buffer cannot be allocated, in place with one exchange
+// Restores the reference's layout (ref_pos_) after relabelling passes moved qubits.  One pass
+// through LDS tiles into the scratch buffer; where that buffer cannot be allocated, in place with one exchange
 // of two index bits per misplaced qubit (at most n - 1, each touching half the state).
 void Engine::canonicalize() {
   run();
@@ -578,18 +578,44 @@ void Engine::canonicalize() {
   }
   if (dst) {
     buf_[0] = dst;
-    BitPerm perm{};
-    perm.count = n_;
+    // dst position of every source position, and back
+    std::vector<unsigned> to(n_), from(n_);
     uint64_t zm = 0, zv = 0;
     for (const auto &kv : pos_) {
-      const unsigned from = kv.second, to = ref_pos_.at(kv.first);
-      perm.from[to] = (uint8_t)from;
-      zm |= ((zmask_ >> from) & 1u) << to;
-      zv |= ((zval_ >> from) & 1u) << to;
+      const unsigned f = kv.second, t = ref_pos_.at(kv.first);
+      to[f] = t;
+      from[t] = f;
+      zm |= ((zmask_ >> f) & 1u) << t;
+      zv |= ((zval_ >> f) & 1u) << t;
     }
+    // tile: the source's lowest k positions (contiguous reads) and the sources of the
+    // destination's lowest k positions (contiguous writes)
+    const unsigned k = std::min(n_, 6u);
+    std::vector<unsigned> tsrc;
+    for (unsigned p = 0; p < k; ++p) tsrc.push_back(p);
+    for (unsigned b = 0; b < k; ++b)
+      if (from[b] >= k) tsrc.push_back(from[b]);
+    std::vector<unsigned> tdst;  // store order: the destination's lowest k first
+    for (unsigned b = 0; b < k; ++b) tdst.push_back(b);
+    for (unsigned p : tsrc)
+      if (to[p] >= k) tdst.push_back(to[p]);
+    PermTile P{};
+    P.m = (uint32_t)tsrc.size();
+    for (size_t r = 0; r < tsrc.size(); ++r) P.src_pos[r] = (uint8_t)tsrc[r];
+    for (size_t r = 0; r < tdst.size(); ++r) {
+      P.dst_pos[r] = (uint8_t)tdst[r];
+      const unsigned srcpos = from[tdst[r]];
+      P.st_from[r] = (uint8_t)(std::find(tsrc.begin(), tsrc.end(), srcpos) - tsrc.begin());
+    }
+    for (unsigned p = 0; p < n_; ++p)
+      if (std::find(tsrc.begin(), tsrc.end(), p) == tsrc.end()) {
+        P.outer_src[P.n_outer] = (uint8_t)p;
+        P.outer_dst[P.n_outer] = (uint8_t)to[p];
+        ++P.n_outer;
+      }
     dispatch([&](auto tag) {
       using R = decltype(tag);
-      dev::gather_bits<R>((const amp_t<R> *)psi_, (amp_t<R> *)dst, amps(), perm, stream_);
+      dev::permute_tiles<R>((const amp_t<R> *)psi_, (amp_t<R> *)dst, amps(), P, stream_);
     });
     std::swap(psi_, buf_[0]);
     sync();

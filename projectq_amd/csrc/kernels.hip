@@ -1008,15 +1008,67 @@ __global__ __launch_bounds__(kBlock) void k_swap_bits(amp_t<R> *__restrict__ psi
   }
 }
 
-// The same layout change in one pass into another buffer: writes in order, reads gathered.
+// The same layout change in one pass into another buffer, through LDS tiles so that both the
+// reads and the writes run along contiguous segments (PermTile).  Every index is GF(2)-linear in
+// its tile bits, so a thread adds one XOR per element to its own offsets.
 template <typename R>
-__global__ __launch_bounds__(kBlock) void k_gather_bits(const amp_t<R> *__restrict__ src, amp_t<R> *__restrict__ dst,
-                                                        uint64_t n, BitPerm perm) {
-  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-  for (uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; j < n; j += stride) {
-    uint64_t i = 0;
-    for (uint32_t b = 0; b < perm.count; ++b) i |= ((j >> b) & 1u) << perm.from[b];
-    dst[j] = src[i];
+__global__ __launch_bounds__(kBlock) void k_permute_tiles(const amp_t<R> *__restrict__ src, amp_t<R> *__restrict__ dst,
+                                                          uint64_t n_tiles, PermTile P) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  amp_t<R> *tile = reinterpret_cast<amp_t<R> *>(smem);
+  constexpr int kTurns = (1 << kPermTileMaxBits) / kBlock;  // 16 turns of 256 elements at most
+  const uint32_t size = 1u << P.m;
+  auto spread = [&](uint32_t x, const uint8_t *pos) {
+    uint64_t v = 0;
+    for (uint32_t r = 0; r < P.m; ++r) v |= uint64_t((x >> r) & 1u) << pos[r];
+    return v;
+  };
+  auto gather_store = [&](uint32_t y) {
+    uint32_t x = 0;
+    for (uint32_t r = 0; r < P.m; ++r) x |= ((y >> r) & 1u) << P.st_from[r];
+    return x;
+  };
+  // Offsets are linear in the tile index: a thread's own part, plus the part of turn j (bits 8..11
+  // of the tile index), the XOR of four uniform values.
+  const uint64_t src_t = spread(threadIdx.x, P.src_pos), dst_t = spread(threadIdx.x, P.dst_pos);
+  const uint32_t lds_t = gather_store(threadIdx.x);
+  uint64_t src_j[4], dst_j[4];
+  uint32_t lds_j[4];
+#pragma unroll
+  for (int b = 0; b < 4; ++b) {
+    src_j[b] = spread(uint32_t(kBlock) << b, P.src_pos);
+    dst_j[b] = spread(uint32_t(kBlock) << b, P.dst_pos);
+    lds_j[b] = gather_store(uint32_t(kBlock) << b);
+  }
+  for (uint64_t tid = blockIdx.x; tid < n_tiles; tid += gridDim.x) {
+    uint64_t sb = 0, db = 0;
+    for (uint32_t i = 0; i < P.n_outer; ++i) {
+      const uint64_t bit = (tid >> i) & 1u;
+      sb |= bit << P.outer_src[i];
+      db |= bit << P.outer_dst[i];
+    }
+    sb |= src_t;
+    db |= dst_t;
+#pragma unroll
+    for (int j = 0; j < kTurns; ++j) {
+      if ((uint32_t)j * kBlock + threadIdx.x < size) {
+        const uint64_t o = ((j & 1) ? src_j[0] : 0) ^ ((j & 2) ? src_j[1] : 0) ^ ((j & 4) ? src_j[2] : 0) ^
+                           ((j & 8) ? src_j[3] : 0);
+        tile[j * kBlock + threadIdx.x] = src[sb ^ o];
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < kTurns; ++j) {
+      if ((uint32_t)j * kBlock + threadIdx.x < size) {
+        const uint64_t o = ((j & 1) ? dst_j[0] : 0) ^ ((j & 2) ? dst_j[1] : 0) ^ ((j & 4) ? dst_j[2] : 0) ^
+                           ((j & 8) ? dst_j[3] : 0);
+        const uint32_t x = lds_t ^ ((j & 1) ? lds_j[0] : 0u) ^ ((j & 2) ? lds_j[1] : 0u) ^ ((j & 4) ? lds_j[2] : 0u) ^
+                           ((j & 8) ? lds_j[3] : 0u);
+        dst[db ^ o] = tile[x];
+      }
+    }
+    __syncthreads();
   }
 }
 
@@ -1565,8 +1617,13 @@ void swap_bits(amp_t<R> *psi, uint64_t n_amps, int a, int b, hipStream_t s) {
 }
 
 template <typename R>
-void gather_bits(const amp_t<R> *src, amp_t<R> *dst, uint64_t n_amps, const BitPerm &perm, hipStream_t s) {
-  hipLaunchKernelGGL(k_gather_bits<R>, dim3(grid_for(n_amps)), dim3(kBlock), 0, s, src, dst, n_amps, perm);
+void permute_tiles(const amp_t<R> *src, amp_t<R> *dst, uint64_t n_amps, const PermTile &P, hipStream_t s) {
+  if (P.m > (uint32_t)kPermTileMaxBits || (uint64_t(1) << P.m) > n_amps || P.m + P.n_outer > 64)
+    throw ValueError("permute_tiles: bad tile");
+  const uint64_t n_tiles = n_amps >> P.m;
+  const uint64_t grid = std::min<uint64_t>(n_tiles, 16384);
+  const size_t lds = sizeof(amp_t<R>) << P.m;
+  hipLaunchKernelGGL(k_permute_tiles<R>, dim3((unsigned)grid), dim3(kBlock), lds, s, src, dst, n_tiles, P);
   QS_HIP(hipGetLastError());
 }
 
@@ -1700,7 +1757,7 @@ template void convert_amps<float, double>(const amp_t<float> *, amp_t<double> *,
                               hipStream_t);                                                       \
   template void drop_bit<R>(const amp_t<R> *, amp_t<R> *, uint64_t, int, int, hipStream_t);       \
   template void swap_bits<R>(amp_t<R> *, uint64_t, int, int, hipStream_t);                          \
-  template void gather_bits<R>(const amp_t<R> *, amp_t<R> *, uint64_t, const BitPerm &, hipStream_t);  \
+  template void permute_tiles<R>(const amp_t<R> *, amp_t<R> *, uint64_t, const PermTile &, hipStream_t); \
   template void pauli_expect<R>(const amp_t<R> *, uint64_t, uint64_t, uint64_t, double *,         \
                                 hipStream_t);                                                     \
   template void pauli_energy<R>(const amp_t<R> *, uint64_t, const PauliTerm *, int, double *,      \

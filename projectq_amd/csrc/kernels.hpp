@@ -116,9 +116,9 @@ void drop_bit(const amp_t<R> *src, amp_t<R> *dst, uint64_t n_out, int p, int val
 // In place: exchange index bits a and b (a != b) of every amplitude of a 2^n state
 template <typename R>
 void swap_bits(amp_t<R> *psi, uint64_t n_amps, int a, int b, hipStream_t s);
-// dst[j] <- src[i], i = the index whose bit perm.from[b] is bit b of j (j < n_amps)
+// dst <- src with its index bits permuted, one pass through LDS tiles (PermTile)
 template <typename R>
-void gather_bits(const amp_t<R> *src, amp_t<R> *dst, uint64_t n_amps, const BitPerm &perm, hipStream_t s);
+void permute_tiles(const amp_t<R> *src, amp_t<R> *dst, uint64_t n_amps, const PermTile &P, hipStream_t s);
 
 // Both expectation kernels take Hermitian strings only, omega X^x Z^z with omega^2 =
 // (-1)^popcount(x & z) (pauli_hermitian); the caller drops the others (their real part is zero).

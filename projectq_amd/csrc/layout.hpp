@@ -16,10 +16,17 @@ struct BitInsert {
   uint8_t pos[kMaxBits];
 };
 
-// A permutation of index bits: bit b of a destination index is bit from[b] of the source index.
-struct BitPerm {
-  uint32_t count;
-  uint8_t from[kMaxBits];
+// One pass of an index-bit permutation through LDS tiles.  A tile holds the 2^m amplitudes whose
+// source bits outside `src_pos` are fixed; it is loaded with tile bit r at source position
+// src_pos[r] (the first bits are the source's lowest positions: contiguous reads) and stored with
+// tile bit r at destination position dst_pos[r] (the first bits are the destination's lowest
+// positions: contiguous writes), store tile bit r taking load tile bit st_from[r].  The tile id's
+// bits go to the source positions outer_src[i] and the destination positions outer_dst[i].
+constexpr int kPermTileMaxBits = 12;
+struct PermTile {
+  uint32_t m, n_outer;
+  uint8_t src_pos[kPermTileMaxBits], dst_pos[kPermTileMaxBits], st_from[kPermTileMaxBits];
+  uint8_t outer_src[kMaxBits], outer_dst[kMaxBits];
 };
 
 
