@@ -265,6 +265,8 @@ int qs_batch_run(qs_batch *batch, int use_graph, double *energies, double *gpu_m
 /* Tile shape in use (after autotuning) and the number of pass launches per run. */
 int qs_batch_shape(qs_batch *batch, uint32_t *tile_bits, uint32_t *reg_bits, uint32_t *low_bits,
                    uint32_t *launches);
+/* Whether the chosen shape relabels qubits at its tile stores (the energies follow the layout). */
+int qs_batch_relabel(qs_batch *batch, int *relabel);
 void qs_batch_destroy(qs_batch *batch);
 
 #ifdef __cplusplus

@@ -1024,6 +1024,10 @@ API int qs_batch_shape(qs_batch *, uint32_t *t, uint32_t *rb, uint32_t *low, uin
   *t = *rb = *low = *launches = 0;
   return QS_OK;
 }
+API int qs_batch_relabel(qs_batch *, int *relabel) {
+  *relabel = 0;
+  return QS_OK;
+}
 API void qs_batch_destroy(qs_batch *b) { delete b; }
 
 }  // extern "C"

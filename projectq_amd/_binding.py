@@ -181,6 +181,7 @@ _SIGNATURES = {
         ctypes.c_int,
         [ctypes.c_void_p, _u32p, _u32p, _u32p, _u32p],
     ),
+    "qs_batch_relabel": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int)]),
     "qs_batch_destroy": (None, [ctypes.c_void_p]),
     "qs_device_state": (
         ctypes.c_int,
@@ -853,7 +854,10 @@ class CAbiBatch:
     def shape(self):
         t, rb, low, launches = (ctypes.c_uint32() for _ in range(4))
         self._check(self._lib.qs_batch_shape(self._handle, ctypes.byref(t), ctypes.byref(rb), ctypes.byref(low), ctypes.byref(launches)))
-        return {"tile_bits": t.value, "reg_bits": rb.value, "low_bits": low.value, "launches": launches.value}
+        relabel = ctypes.c_int()
+        self._check(self._lib.qs_batch_relabel(self._handle, ctypes.byref(relabel)))
+        return {"tile_bits": t.value, "reg_bits": rb.value, "low_bits": low.value, "launches": launches.value,
+                "relabel": bool(relabel.value)}
 
     def __del__(self):
         handle = getattr(self, "_handle", None)

@@ -1,4 +1,5 @@
 // Batched circuits (see batch.hpp).
+#include <cstdlib>
 #include "batch.hpp"
 
 #include <algorithm>
@@ -20,8 +21,10 @@ T *to_device(const std::vector<T> &v) {
 }
 
 std::vector<BatchShape> candidate_shapes(bool c64) {
-  if (c64) return {{12, 4, 5}, {11, 4, 5}, {10, 4, 4}, {12, 5, 5}, {10, 5, 5}};
-  return {{12, 4, 4}, {11, 4, 4}, {10, 4, 4}};
+  if (c64)
+    return {{12, 4, 5, false}, {11, 4, 5, false}, {10, 4, 4, false}, {12, 5, 5, false},
+            {10, 5, 5, false}, {12, 4, 4, true}, {11, 4, 4, true}, {10, 4, 4, true}};
+  return {{12, 4, 4, false}, {11, 4, 4, false}, {10, 4, 4, false}, {12, 4, 4, true}};
 }
 
 }  // namespace
@@ -39,39 +42,55 @@ Batch::Batch(bool c64, std::vector<BatchCircuit> circuits, bool autotune) : c64_
     for (auto &t : c.terms)
       if ((t.x | t.z) >> c.n) throw ValueError("batch: Pauli string outside its circuit");
   }
-  dev::prepare_tile_kernels();
-  QS_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
-  QS_HIP(hipEventCreate(&ev0_));
-  QS_HIP(hipEventCreate(&ev1_));
-  QS_HIP(hipHostMalloc((void **)&h_energies_, circ_.size() * sizeof(double), hipHostMallocDefault));
   std::vector<BatchShape> shapes = candidate_shapes(c64_);
-  if (!autotune) {
-    build(shapes[0]);
-    return;
+  long forced = -1;  // PROJECTQ_AMD_BATCH_SHAPE=k: candidate k only, no timing (tests)
+  if (const char *env = std::getenv("PROJECTQ_AMD_BATCH_SHAPE")) {
+    forced = std::strtol(env, nullptr, 10);
+    if (forced < 0 || forced >= (long)shapes.size())
+      throw ValueError("PROJECTQ_AMD_BATCH_SHAPE: no such candidate shape");
   }
-  // autotuner: time every candidate shape on this batch (graph launches), keep the fastest
-  std::vector<double> e(circ_.size());
-  double best = std::numeric_limits<double>::infinity();
-  BatchShape best_shape = shapes[0];
-  for (const BatchShape &s : shapes) {
-    build(s);
-    run(true, e.data());  // warm-up
-    double ms = 0.0;
-    for (int r = 0; r < 3; ++r) ms += run(true, e.data());
-    if (ms < best) {
-      best = ms;
-      best_shape = s;
+  try {  // a constructor that throws runs no destructor: free what was made so far
+    dev::prepare_tile_kernels();
+    QS_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+    QS_HIP(hipEventCreate(&ev0_));
+    QS_HIP(hipEventCreate(&ev1_));
+    QS_HIP(hipHostMalloc((void **)&h_energies_, circ_.size() * sizeof(double), hipHostMallocDefault));
+    if (forced >= 0 || !autotune) {
+      build(shapes[forced >= 0 ? (size_t)forced : 0]);
+      return;
     }
+    // autotuner: time every candidate shape on this batch (graph launches), keep the fastest
+    std::vector<double> e(circ_.size());
+    double best = std::numeric_limits<double>::infinity();
+    BatchShape best_shape = shapes[0];
+    for (const BatchShape &s : shapes) {
+      build(s);
+      run(true, e.data());  // warm-up
+      double ms = 0.0;
+      for (int r = 0; r < 3; ++r) ms += run(true, e.data());
+      if (ms < best) {
+        best = ms;
+        best_shape = s;
+      }
+    }
+    build(best_shape);
+  } catch (...) {
+    destroy();
+    throw;
   }
-  build(best_shape);
 }
 
-Batch::~Batch() {
+Batch::~Batch() { destroy(); }
+
+void Batch::destroy() {
   release();
   if (ev0_) (void)hipEventDestroy(ev0_);
   if (ev1_) (void)hipEventDestroy(ev1_);
-  (void)hipHostFree(h_energies_);
+  if (h_energies_) (void)hipHostFree(h_energies_);
   if (stream_) (void)hipStreamDestroy(stream_);
+  ev0_ = ev1_ = nullptr;
+  h_energies_ = nullptr;
+  stream_ = nullptr;
 }
 
 void Batch::release() {
@@ -108,6 +127,7 @@ void Batch::build(const BatchShape &shape) {
   std::vector<uint64_t> state_off(circ_.size());
   std::vector<std::vector<dev::BatchJob>> jobs;  // by pass index
   std::vector<std::vector<size_t>> job_circuit;  // circuit of every job, by pass index
+  std::vector<std::vector<uint32_t>> final_pos(circ_.size());  // label -> position after the passes
   total_amps_ = 0;
   for (size_t c = 0; c < circ_.size(); ++c) {
     const BatchCircuit &bc = circ_[c];
@@ -119,10 +139,20 @@ void Batch::build(const BatchShape &shape) {
     gates = merge_single_qubit_runs(gates);
     const unsigned t = std::min(bc.n, shape.tmax);
     TileShape ts{bc.n, t, shape.lmin};
-    std::vector<PPass> passes = schedule_tiles(gates, ts);
+    std::vector<PPass> passes = schedule_tiles(gates, ts, nullptr, 256, 16.0, shape.relabel);
+    std::vector<uint32_t> &phys = final_pos[c];
+    std::vector<uint32_t> label(bc.n);
+    phys.resize(bc.n);
+    for (uint32_t q = 0; q < bc.n; ++q) phys[q] = label[q] = q;
     for (size_t p = 0; p < passes.size(); ++p) {
       TileProgram prog;
       build_tile_pass(gates, passes[p], t, shape.rb, c64_, prog);
+      for (const auto &sw : prog.swaps) {
+        const uint32_t a = label[sw.first], b = label[sw.second];
+        std::swap(label[sw.first], label[sw.second]);
+        phys[a] = sw.second;
+        phys[b] = sw.first;
+      }
       if (prog.stages.empty()) continue;
       dev::BatchJob job{};
       job.state_off = state_off[c];
@@ -177,7 +207,14 @@ void Batch::build(const BatchShape &shape) {
     slots += e.n_blocks;
     e.term_off = (uint32_t)terms.size();
     e.n_terms = (uint32_t)circ_[c].terms.size();
-    for (const dev::PauliTerm &t : circ_[c].terms) {
+    for (const dev::PauliTerm &t0 : circ_[c].terms) {
+      // the strings act on the circuit's final layout (relabelling moves qubits)
+      dev::PauliTerm t = t0;
+      t.x = t.z = 0;
+      for (uint32_t q = 0; q < circ_[c].n; ++q) {
+        t.x |= ((t0.x >> q) & 1u) << final_pos[c][q];
+        t.z |= ((t0.z >> q) & 1u) << final_pos[c][q];
+      }
       // (-1)^{popcount((i ^ x) & z)} = (-1)^{popcount(i & z)} (-1)^{popcount(x & z)}
       double s = (__builtin_popcountll(t.x & t.z) & 1) ? -1.0 : 1.0;
       // pairs (i, i ^ x): half the amplitudes, twice the weight

@@ -25,6 +25,7 @@ struct BatchCircuit {
 
 struct BatchShape {
   unsigned tmax = 12, rb = 4, lmin = 5;
+  bool relabel = false;  // tile stores relabel qubits (schedule_tiles); energies follow the layout
 };
 
 class Batch {
@@ -43,6 +44,7 @@ class Batch {
  private:
   void build(const BatchShape &shape);
   void release();
+  void destroy();  // release() plus the stream, events and pinned results
   void enqueue();  // every kernel of one run on stream_
 
   bool c64_;

@@ -347,6 +347,10 @@ API int qs_batch_shape(qs_batch *b, uint32_t *t, uint32_t *rb, uint32_t *lmin, u
   });
 }
 
+API int qs_batch_relabel(qs_batch *b, int *relabel) {
+  return guarded(nullptr, [&] { *relabel = b->batch.shape().relabel ? 1 : 0; });
+}
+
 API void qs_batch_destroy(qs_batch *b) {
   if (!b) return;
   (void)hipSetDevice(b->device);

@@ -2,6 +2,8 @@
 results (complex128 to 1e-10, complex64 to 2e-5 relative to sum |c_t|); the hipGraph launch,
 eager launches and autotuned shapes must all give the same answers."""
 
+import os
+
 import numpy as np
 import pytest
 
@@ -74,3 +76,21 @@ def test_batch_shapes_agree():
     a, _ = BatchExpectation(circ, ops, precision="c128", autotune=False).run()
     b, _ = BatchExpectation(circ, ops, precision="c128", autotune=True).run()
     assert np.max(np.abs(a - b)) <= 1e-12
+    # every candidate on its own, including the shapes whose tile stores relabel qubits (the
+    # Pauli strings then act on each circuit's final layout)
+    relabelled = 0
+    for k in range(8):
+        os.environ["PROJECTQ_AMD_BATCH_SHAPE"] = str(k)
+        try:
+            for prec, tol in (("c128", 1e-12), ("c64", 2e-5)):
+                try:
+                    batch = BatchExpectation(circ, ops, precision=prec, autotune=False)
+                except ValueError:
+                    continue  # fewer candidates for this precision
+                e, _ = batch.run()
+                scale = np.array([sum(abs(c) for _, c in op) for op in ops])
+                assert np.all(np.abs(e - a) <= tol * scale), (k, prec, batch.shape())
+                relabelled += batch.shape()["relabel"]
+        finally:
+            del os.environ["PROJECTQ_AMD_BATCH_SHAPE"]
+    assert relabelled >= 2
