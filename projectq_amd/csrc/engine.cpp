@@ -82,6 +82,11 @@ Engine::Engine(uint32_t seed, int precision, int fusion_mode, int planner, unsig
     else if (!std::strcmp(env, "1")) mfma_mode_ = 1;
     else throw ValueError("PROJECTQ_AMD_MFMA must be 0, 1 or all");
   }
+  if (const char *env = std::getenv("PROJECTQ_AMD_CZ")) {
+    if (!std::strcmp(env, "0")) cnot_cz_ = false;
+    else if (!std::strcmp(env, "1")) cnot_cz_ = true;
+    else throw ValueError("PROJECTQ_AMD_CZ must be 0 or 1");
+  }
   if (const char *env = std::getenv("PROJECTQ_AMD_RELABEL")) {
     if (!std::strcmp(env, "0")) relabel_mode_ = 0;
     else if (!std::strcmp(env, "1")) relabel_mode_ = 1;
@@ -513,7 +518,7 @@ std::vector<PGate> Engine::resolve(const std::vector<Gate> &gates) const {
     if (p.tmask() & p.cmask) continue;  // a control that is also a target: never applies
     pg.push_back(std::move(p));
   }
-  return merge_single_qubit_runs(pg);
+  return merge_single_qubit_runs(cnot_cz_ ? cnot_to_cz(pg) : pg);
 }
 
 // Tile size: 2^t amplitudes of LDS per workgroup, at most n bits, and small enough that a state
@@ -787,6 +792,11 @@ void Engine::exec_tile_program(const TileProgram &prog, unsigned rb) {
   const dev::TileStage *d_st = (const dev::TileStage *)up(prog.stages);
   const dev::RegOp *d_rops = (const dev::RegOp *)up(prog.rops);
   const dev::TileOp *d_tops = (const dev::TileOp *)up(prog.tops);
+  bool two_qubit_ops = false;
+  for (const dev::RegOp &op : prog.rops) {
+    const uint32_t kind = (op.code >> 8) & 15u;
+    two_qubit_ops = two_qubit_ops || kind == dev::kOp2General || kind == dev::kOp2Diag;
+  }
   PassFeatures f{};
   if (recording_) f = tile_pass_features(prog, n_, c64_, rb);
   dispatch([&](auto tag) {
@@ -799,7 +809,7 @@ void Engine::exec_tile_program(const TileProgram &prog, unsigned rb) {
     const amp_t<R> *dm = (const amp_t<R> *)up(m);
     hipEvent_t e0, e1;
     begin_pass(&e0, &e1);
-    dev::tile_pass<R>((amp_t<R> *)psi_, n_tiles, prog.P, (int)rb, d_st, d_rops, d_tops, dm, stream_);
+    dev::tile_pass<R>((amp_t<R> *)psi_, n_tiles, prog.P, (int)rb, d_st, d_rops, d_tops, dm, two_qubit_ops, stream_);
     end_pass(e0, e1, recording_ ? &f : nullptr);
   });
   count_pass(2.0 * (double)amp_bytes() * (double)amps());

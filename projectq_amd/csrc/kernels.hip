@@ -337,6 +337,23 @@ __device__ __forceinline__ void rop_x(R *vr, R *vi, const amp_t<R> *__restrict__
     vi[j] = xi;
   }
 }
+// Z (diag(1, -1)): the amplitudes with bit B set change sign, in place (an XOR of the sign bit;
+// predicated lanes XOR with zero).  A CNOT between 1-qubit gates on its target reaches the tile
+// kernels as H, controlled Z, H with the H's merged into those gates (cnot_to_cz in planner.cpp).
+template <typename R>
+__device__ __forceinline__ R flip_sign(R x, bool act) {
+  if constexpr (sizeof(R) == 4) return __uint_as_float(__float_as_uint(x) ^ (act ? 0x80000000u : 0u));
+  else return __longlong_as_double(__double_as_longlong(x) ^ (act ? (1ll << 63) : 0ll));
+}
+template <typename R, int RB, bool PRED, int C, int B>
+__device__ __forceinline__ void rop_z(R *vr, R *vi, const amp_t<R> *__restrict__, bool act) {
+#pragma unroll
+  for (int i = 0; i < (1 << RB); ++i) {
+    if (!(i & (1 << B)) || !reg_ctrl_ok(i, C)) continue;
+    vr[i] = flip_sign<R>(vr[i], PRED ? act : true);
+    vi[i] = flip_sign<R>(vi[i], PRED ? act : true);
+  }
+}
 template <typename R, int RB, bool PRED, int C, int B0, int B1>
 __device__ __forceinline__ void rop_general2(R *vr, R *vi, const amp_t<R> *__restrict__ m, bool act) {
   R mr[16], mi[16];
@@ -435,17 +452,25 @@ __device__ __forceinline__ void rop_diag2(R *vr, R *vi, const amp_t<R> *__restri
     default: QS_R2C(FN, 3, 4) break;                             \
   }
 
-template <typename R, int RB, bool PRED>
+// TWO = false: a kernel for passes with no 2-qubit register op (the host picks it per pass); the
+// dispatch then holds only the 1-qubit forms, and the register allocator works on far fewer arms.
+template <typename R, int RB, bool PRED, bool TWO = true>
 __device__ __forceinline__ void reg_op(uint32_t code, R *vr, R *vi, const amp_t<R> *__restrict__ m, bool act) {
   const uint32_t kind = (code >> 8) & 15u, b0 = code & 15u, b1 = (code >> 4) & 15u, cb = code >> 12;
+  (void)b1;
   switch (kind) {
     case kOp1General: QS_R1B(rop_general) break;
     case kOp1Real: QS_R1B(rop_real) break;
     case kOp1Diag: QS_R1B(rop_diag) break;
     case kOp1Anti: QS_R1B(rop_anti) break;
     case kOp1X: QS_R1B(rop_x) break;
-    case kOp2General: QS_R2B(rop_general2) break;
-    default: QS_R2B(rop_diag2) break;
+    case kOp1Z: QS_R1B(rop_z) break;
+    default:
+      if constexpr (TWO) {
+        if (kind == kOp2General) QS_R2B(rop_general2)
+        else QS_R2B(rop_diag2)
+      }
+      break;
   }
 }
 #undef QS_R1
@@ -457,21 +482,21 @@ __device__ __forceinline__ void reg_op(uint32_t code, R *vr, R *vi, const amp_t<
 
 // One register op: global controls are uniform over the block (a branch); tile-bit controls are
 // per lane (the predicated form).
-template <typename R, int RB>
+template <typename R, int RB, bool TWO = true>
 __device__ __forceinline__ void reg_apply(const RegOp *__restrict__ op, uint64_t base, uint32_t lb, R *vr, R *vi,
                                           const amp_t<R> *__restrict__ mats) {
   const uint64_t gc = op->gcmask;
   const uint32_t lc = op->lcmask;
   if ((base & gc) == gc) {
-    if (lc == 0) reg_op<R, RB, false>(op->code, vr, vi, mats + op->mat, true);
-    else reg_op<R, RB, true>(op->code, vr, vi, mats + op->mat, (lb & lc) == lc);
+    if (lc == 0) reg_op<R, RB, false, TWO>(op->code, vr, vi, mats + op->mat, true);
+    else reg_op<R, RB, true, TWO>(op->code, vr, vi, mats + op->mat, (lb & lc) == lc);
   }
 }
 
 // TURN ops per turn of the op loop.  With 2 the register allocator moves the amplitudes between
 // register sets once per turn rather than once per op (worth about 2% on the pass kernel), at
 // the price of a second copy of the op dispatch; kernels off the hot path use 1.
-template <typename R, int RB, int TURN, int NT = kBlock>
+template <typename R, int RB, int TURN, int NT = kBlock, bool TWO = true>
 __device__ __forceinline__ void reg_stage(amp_t<R> *__restrict__ tile, uint32_t t, uint64_t base,
                                           const TileStage *__restrict__ st,
                                           const RegOp *__restrict__ rops,
@@ -499,9 +524,9 @@ __device__ __forceinline__ void reg_stage(amp_t<R> *__restrict__ tile, uint32_t 
       vi[i] = a.y;
     }
     for (uint32_t o = ob; o < oe; o += TURN) {
-      reg_apply<R, RB>(rops + o, base, lb, vr, vi, mats);
+      reg_apply<R, RB, TWO>(rops + o, base, lb, vr, vi, mats);
       if constexpr (TURN == 2) {
-        if (o + 1 < oe) reg_apply<R, RB>(rops + o + 1, base, lb, vr, vi, mats);
+        if (o + 1 < oe) reg_apply<R, RB, TWO>(rops + o + 1, base, lb, vr, vi, mats);
       }
     }
     // Recompute the LDS addresses for the write-back instead of keeping 2^RB of them live across
@@ -623,14 +648,14 @@ __device__ __forceinline__ uint32_t store_source(uint32_t x, const TilePass &P) 
 }
 
 // The stages of a pass on one tile resident in LDS, a workgroup barrier after each.
-template <typename R, int RB, int TURN, int NT = kBlock>
+template <typename R, int RB, int TURN, int NT = kBlock, bool TWO = true>
 __device__ __forceinline__ void run_stages(amp_t<R> *__restrict__ tile, uint32_t t, uint64_t base, uint32_t n_stages,
                                            const TileStage *__restrict__ stages, const RegOp *__restrict__ rops,
                                            const TileOp *__restrict__ tops, const amp_t<R> *__restrict__ mats) {
   for (uint32_t s = 0; s < n_stages; ++s) {
     const TileStage *st = stages + s;
     if (st->kind == kStageRegisters) {
-      reg_stage<R, RB, TURN, NT>(tile, t, base, st, rops, mats);
+      reg_stage<R, RB, TURN, NT, TWO>(tile, t, base, st, rops, mats);
     } else {
       const TileOp *op = tops + st->begin;
       const uint64_t gc = op->gcmask;
@@ -734,7 +759,7 @@ __global__ __launch_bounds__(kBlock) void k_tile_pass(amp_t<R> *__restrict__ psi
 // (2^t = kStreamChunk * NT * VN), so all streaming is unguarded straight-line code.  NT = 256
 // serves 2^12-amplitude complex64 tiles (32 KiB of LDS, four workgroups per CU), NT = 512 tiles of
 // 2^13 (64 KiB, two workgroups per CU): the same per-thread work and the same waves per CU.
-template <typename R, int RB, int NT>
+template <typename R, int RB, int NT, bool TWO>
 __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_tile_pass_pf(amp_t<R> *__restrict__ psi, uint64_t n_tiles,
                                                          TilePass P, const TileStage *__restrict__ stages,
                                                          const RegOp *__restrict__ rops,
@@ -790,7 +815,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_
   };
   if (next < n_tiles) prefetch(nbase);
   while (true) {
-    run_stages<R, RB, 2, NT>(tile, t, base, P.n_stages, stages, rops, tops, mats);
+    run_stages<R, RB, 2, NT, TWO>(tile, t, base, P.n_stages, stages, rops, tops, mats);
     uint32_t ls = l0;
     asm volatile("" : "+v"(ls));
 #pragma unroll
@@ -1447,7 +1472,8 @@ void apply_block(amp_t<R> *psi, uint64_t n_groups, const BitInsert &ins, uint64_
 
 template <typename R, int RB>
 void launch_tile_pass(amp_t<R> *psi, uint64_t n_tiles, const TilePass &P, const TileStage *d_stages,
-                      const RegOp *d_rops, const TileOp *d_tops, const amp_t<R> *d_mats, hipStream_t s) {
+                      const RegOp *d_rops, const TileOp *d_tops, const amp_t<R> *d_mats, bool two_qubit_ops,
+                      hipStream_t s) {
   prepare_tile_kernels();
   const size_t lds = (sizeof(amp_t<R>) << P.t) + (sizeof(uint64_t) << P.h);
   // workgroups per launch: each loops over n_tiles / grid tiles.  About 16 tiles per workgroup
@@ -1468,12 +1494,20 @@ void launch_tile_pass(amp_t<R> *psi, uint64_t n_tiles, const TilePass &P, const 
   bool done = false;
   if constexpr (RB == 4 && sizeof(R) == 4) {  // the only instantiations of the pipelined kernel
     if (prefetch && tsize == chunk) {
-      hipLaunchKernelGGL((k_tile_pass_pf<R, RB, kBlock>), dim3((unsigned)grid), dim3(kBlock), lds, s, psi, n_tiles, P,
-                         d_stages, d_rops, d_tops, d_mats);
+      if (two_qubit_ops)
+        hipLaunchKernelGGL((k_tile_pass_pf<R, RB, kBlock, true>), dim3((unsigned)grid), dim3(kBlock), lds, s, psi,
+                           n_tiles, P, d_stages, d_rops, d_tops, d_mats);
+      else
+        hipLaunchKernelGGL((k_tile_pass_pf<R, RB, kBlock, false>), dim3((unsigned)grid), dim3(kBlock), lds, s, psi,
+                           n_tiles, P, d_stages, d_rops, d_tops, d_mats);
       done = true;
     } else if (prefetch && tsize == 2 * chunk) {
-      hipLaunchKernelGGL((k_tile_pass_pf<R, RB, 2 * kBlock>), dim3((unsigned)grid), dim3(2 * kBlock), lds, s, psi,
-                         n_tiles, P, d_stages, d_rops, d_tops, d_mats);
+      if (two_qubit_ops)
+        hipLaunchKernelGGL((k_tile_pass_pf<R, RB, 2 * kBlock, true>), dim3((unsigned)grid), dim3(2 * kBlock), lds, s,
+                           psi, n_tiles, P, d_stages, d_rops, d_tops, d_mats);
+      else
+        hipLaunchKernelGGL((k_tile_pass_pf<R, RB, 2 * kBlock, false>), dim3((unsigned)grid), dim3(2 * kBlock), lds,
+                           s, psi, n_tiles, P, d_stages, d_rops, d_tops, d_mats);
       done = true;
     }
   }
@@ -1489,11 +1523,11 @@ void launch_tile_pass(amp_t<R> *psi, uint64_t n_tiles, const TilePass &P, const 
 
 template <typename R>
 void tile_pass(amp_t<R> *psi, uint64_t n_tiles, const TilePass &P, int rb, const TileStage *d_stages,
-               const RegOp *d_rops, const TileOp *d_tops, const amp_t<R> *d_mats, hipStream_t s) {
+               const RegOp *d_rops, const TileOp *d_tops, const amp_t<R> *d_mats, bool two_qubit_ops, hipStream_t s) {
   if (P.h > (uint32_t)kTileMaxHigh || P.L < 1 || P.t > (uint32_t)kTileMaxBits)
     throw ValueError("tile pass: bad tile shape");
-  if (rb == 5) launch_tile_pass<R, 5>(psi, n_tiles, P, d_stages, d_rops, d_tops, d_mats, s);
-  else if (rb == 4) launch_tile_pass<R, 4>(psi, n_tiles, P, d_stages, d_rops, d_tops, d_mats, s);
+  if (rb == 5) launch_tile_pass<R, 5>(psi, n_tiles, P, d_stages, d_rops, d_tops, d_mats, two_qubit_ops, s);
+  else if (rb == 4) launch_tile_pass<R, 4>(psi, n_tiles, P, d_stages, d_rops, d_tops, d_mats, two_qubit_ops, s);
   else throw ValueError("tile pass: register bits must be 4 or 5");
 }
 
@@ -1512,8 +1546,10 @@ void prepare_tile_kernels() {
   QS_HIP(hipFuncSetAttribute((const void *)k_tile_pass<float, 5, false>, hipFuncAttributeMaxDynamicSharedMemorySize, kTileLdsMax<float>));
   QS_HIP(hipFuncSetAttribute((const void *)k_tile_pass<double, 4, false>, hipFuncAttributeMaxDynamicSharedMemorySize, kTileLdsMax<double>));
   QS_HIP(hipFuncSetAttribute((const void *)k_tile_pass<double, 5, false>, hipFuncAttributeMaxDynamicSharedMemorySize, kTileLdsMax<double>));
-  QS_HIP(hipFuncSetAttribute((const void *)k_tile_pass_pf<float, 4, kBlock>, hipFuncAttributeMaxDynamicSharedMemorySize, kTileLdsMax<float>));
-  QS_HIP(hipFuncSetAttribute((const void *)k_tile_pass_pf<float, 4, 2 * kBlock>, hipFuncAttributeMaxDynamicSharedMemorySize, kTileLdsMax<float>));
+  QS_HIP(hipFuncSetAttribute((const void *)k_tile_pass_pf<float, 4, kBlock, true>, hipFuncAttributeMaxDynamicSharedMemorySize, kTileLdsMax<float>));
+  QS_HIP(hipFuncSetAttribute((const void *)k_tile_pass_pf<float, 4, 2 * kBlock, true>, hipFuncAttributeMaxDynamicSharedMemorySize, kTileLdsMax<float>));
+  QS_HIP(hipFuncSetAttribute((const void *)k_tile_pass_pf<float, 4, kBlock, false>, hipFuncAttributeMaxDynamicSharedMemorySize, kTileLdsMax<float>));
+  QS_HIP(hipFuncSetAttribute((const void *)k_tile_pass_pf<float, 4, 2 * kBlock, false>, hipFuncAttributeMaxDynamicSharedMemorySize, kTileLdsMax<float>));
   QS_HIP(hipFuncSetAttribute((const void *)k_batch_tile_pass<float, 4>, hipFuncAttributeMaxDynamicSharedMemorySize, kTileLdsMax<float>));
   QS_HIP(hipFuncSetAttribute((const void *)k_batch_tile_pass<float, 5>, hipFuncAttributeMaxDynamicSharedMemorySize, kTileLdsMax<float>));
   QS_HIP(hipFuncSetAttribute((const void *)k_batch_tile_pass<double, 4>, hipFuncAttributeMaxDynamicSharedMemorySize, kTileLdsMax<double>));
@@ -1741,7 +1777,7 @@ template void convert_amps<float, double>(const amp_t<float> *, amp_t<double> *,
                                    uint32_t, int, const TileStage *, const RegOp *, const TileOp *,\
                                    const amp_t<R> *, hipStream_t);                                \
   template void tile_pass<R>(amp_t<R> *, uint64_t, const TilePass &, int, const TileStage *,      \
-                             const RegOp *, const TileOp *, const amp_t<R> *, hipStream_t);       \
+                             const RegOp *, const TileOp *, const amp_t<R> *, bool, hipStream_t); \
   template void apply_block<R>(amp_t<R> *, uint64_t, const BitInsert &, uint64_t,                 \
                                const uint32_t *, int, const amp_t<R> *, hipStream_t);             \
   template void norm_masked<R>(const amp_t<R> *, uint64_t, const BitInsert &, uint64_t, double *, \

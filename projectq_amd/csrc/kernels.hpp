@@ -37,10 +37,12 @@ struct MathArgs {
   uint8_t pos[kMaxRegs][32];
 };
 
-// tile passes (TilePass / TileStage / RegOp / TileOp in layout.hpp); rb = register bits (4, 5)
+// tile passes (TilePass / TileStage / RegOp / TileOp in layout.hpp); rb = register bits (4, 5);
+// two_qubit_ops: whether any register op of the pass is a 2-qubit one (false lets complex64
+// passes run a kernel without the 2-qubit op forms)
 template <typename R>
 void tile_pass(amp_t<R> *psi, uint64_t n_tiles, const TilePass &P, int rb, const TileStage *d_stages,
-               const RegOp *d_rops, const TileOp *d_tops, const amp_t<R> *d_mats, hipStream_t s);
+               const RegOp *d_rops, const TileOp *d_tops, const amp_t<R> *d_mats, bool two_qubit_ops, hipStream_t s);
 
 // set the dynamic-LDS limit of every tile-pass kernel (call before any graph capture)
 void prepare_tile_kernels();

@@ -65,7 +65,7 @@ struct TileStage {
 //   kind 0: 1-qubit general 2x2 on b0      kind 3: 1-qubit X (swap) on b0
 //   kind 1: 1-qubit diagonal on b0         kind 4: 2-qubit general 4x4 on (b0, b1)
 //   kind 2: 1-qubit anti-diagonal on b0    kind 5: 2-qubit diagonal on (b0, b1)
-//   kind 6: 1-qubit real 2x2 on b0
+//   kind 6: 1-qubit real 2x2 on b0         kind 7: 1-qubit Z (diag(1, -1): sign flip) on b0
 // Matrix bit 0 belongs to b0 and bit 1 to b1.
 enum : uint32_t {
   kOp1General = 0,
@@ -74,7 +74,8 @@ enum : uint32_t {
   kOp1X = 3,
   kOp2General = 4,
   kOp2Diag = 5,
-  kOp1Real = 6
+  kOp1Real = 6,
+  kOp1Z = 7
 };
 struct RegOp {
   uint32_t code;

@@ -37,6 +37,61 @@ std::vector<PGate> merge_single_qubit_runs(const std::vector<PGate> &gates) {
   return out;
 }
 
+std::vector<PGate> cnot_to_cz(const std::vector<PGate> &gates) {
+  const std::complex<double> zero(0.0, 0.0), one(1.0, 0.0);
+  auto uncontrolled_single_on = [&](int i, uint32_t p) {
+    return i >= 0 && i < (int)gates.size() && gates[i].tpos.size() == 1 && gates[i].tpos[0] == p &&
+           gates[i].cmask == 0;
+  };
+  // previous and next gate touching each gate's target position
+  const int n = (int)gates.size();
+  std::vector<int> prev(n, -1), next(n, n);
+  {
+    std::map<uint32_t, int> last;
+    for (int i = 0; i < n; ++i) {
+      const uint64_t touch = gates[i].tmask() | gates[i].cmask;
+      if (gates[i].tpos.size() == 1) {
+        auto it = last.find(gates[i].tpos[0]);
+        if (it != last.end()) prev[i] = it->second;
+      }
+      for (uint32_t p = 0; p < 64; ++p)
+        if ((touch >> p) & 1) last[p] = i;
+    }
+    last.clear();
+    for (int i = n - 1; i >= 0; --i) {
+      const uint64_t touch = gates[i].tmask() | gates[i].cmask;
+      if (gates[i].tpos.size() == 1) {
+        auto it = last.find(gates[i].tpos[0]);
+        if (it != last.end()) next[i] = it->second;
+      }
+      for (uint32_t p = 0; p < 64; ++p)
+        if ((touch >> p) & 1) last[p] = i;
+    }
+  }
+  const double h = std::sqrt(0.5);
+  const std::vector<std::complex<double>> H = {h, h, h, -h}, Z = {one, zero, zero, -one};
+  std::vector<PGate> out;
+  out.reserve(gates.size() + gates.size() / 2);
+  for (int i = 0; i < n; ++i) {
+    const PGate &g = gates[i];
+    const bool cnot = g.tpos.size() == 1 && g.cmask != 0 && g.m.size() == 4 && g.m[0] == zero && g.m[1] == one &&
+                      g.m[2] == one && g.m[3] == zero;
+    if (cnot && uncontrolled_single_on(prev[i], g.tpos[0]) && uncontrolled_single_on(next[i], g.tpos[0])) {
+      PGate a, z, b;
+      a.tpos = b.tpos = z.tpos = g.tpos;
+      a.m = b.m = H;
+      z.m = Z;
+      z.cmask = g.cmask;
+      out.push_back(std::move(a));
+      out.push_back(std::move(z));
+      out.push_back(std::move(b));
+    } else {
+      out.push_back(g);
+    }
+  }
+  return out;
+}
+
 std::vector<uint32_t> choose_by_score(const std::vector<SlotCandidate> &cands, unsigned slots) {
   std::vector<SlotCandidate> c = cands;
   std::stable_sort(c.begin(), c.end(), [](const SlotCandidate &a, const SlotCandidate &b) {
@@ -239,7 +294,7 @@ struct LocalGate {
 uint32_t classify1(const std::vector<cplx> &m) {
   const cplx zero(0.0, 0.0), one(1.0, 0.0);
   if (m[0] == zero && m[3] == zero) return (m[1] == one && m[2] == one) ? dev::kOp1X : dev::kOp1Anti;
-  if (m[1] == zero && m[2] == zero) return dev::kOp1Diag;
+  if (m[1] == zero && m[2] == zero) return (m[0] == one && m[3] == -one) ? dev::kOp1Z : dev::kOp1Diag;
   bool real = true;
   for (int i = 0; i < 4; ++i) real = real && m[i].imag() == 0.0;
   return real ? dev::kOp1Real : dev::kOp1General;
@@ -604,7 +659,9 @@ PassFeatures tile_pass_features(const TileProgram &prog, unsigned n, bool c64, u
   float reg = 0, lds = 0, kinds[7] = {0, 0, 0, 0, 0, 0, 0}, rctl = 0, lctl = 0, gctl = 0, ldsflops = 0;
   for (const auto &st : prog.stages) (st.kind == dev::kStageRegisters ? reg : lds) += 1;
   for (const auto &op : prog.rops) {
-    kinds[std::min<uint32_t>((op.code >> 8) & 15u, 6)] += 1;
+    uint32_t kd = (op.code >> 8) & 15u;
+    if (kd == dev::kOp1Z) kd = dev::kOp1Diag;  // a diagonal op (cheaper than the general diagonal)
+    kinds[std::min<uint32_t>(kd, 6)] += 1;
     rctl += op.rcmask != 0;
     lctl += op.lcmask != 0;
     gctl += op.gcmask != 0;

@@ -60,6 +60,13 @@ struct SlotCandidate {
 using SlotChooser =
     std::function<std::vector<uint32_t>(const std::vector<SlotCandidate> &cands, unsigned slots)>;
 
+// A CNOT (controlled X, any controls) whose target's previous and next gates are uncontrolled
+// 1-qubit gates on that target becomes H, controlled Z, H: the same unitary, and after
+// merge_single_qubit_runs the H's are part of those neighbours.  The tile kernels apply the
+// controlled Z as a sign flip in place (kOp1Z) instead of exchanging registers, and the
+// scheduler may move a diagonal gate past gates that only read its target.
+std::vector<PGate> cnot_to_cz(const std::vector<PGate> &gates);
+
 // Merge runs of uncontrolled single-qubit gates on the same position (no other gate touching
 // that position in between) into one matrix product.
 std::vector<PGate> merge_single_qubit_runs(const std::vector<PGate> &gates);

@@ -39,13 +39,18 @@ void apply_small(std::vector<cplx> &v, uint32_t code, uint32_t rcmask, const cpl
   const uint32_t kind = (code >> 8) & 15u, b0 = code & 15u, b1 = (code >> 4) & 15u, cb = code >> 12;
   // the kernels read the register control from the code; rcmask must agree with it
   check(cb <= rb && (cb == 0 ? rcmask == 0 : rcmask == (1u << (cb - 1))), "register control encoding");
-  check(cb == 0 || (cb - 1 != b0 && (kind < 4 || kind == qs::dev::kOp1Real || cb - 1 != b1)), "control on a target");
+  const bool one_qubit = kind < 4 || kind == qs::dev::kOp1Real || kind == qs::dev::kOp1Z;
+  check(cb == 0 || (cb - 1 != b0 && (one_qubit || cb - 1 != b1)), "control on a target");
   const uint32_t rc = cb ? 1u << (cb - 1) : 0u;
   const uint32_t NR = 1u << rb;
-  check(b0 < rb && (kind < 4 || kind == qs::dev::kOp1Real || (b1 < rb && b0 < b1)), "register op bits");
-  if (kind < 4 || kind == qs::dev::kOp1Real) {
+  check(b0 < rb && (one_qubit || (b1 < rb && b0 < b1)), "register op bits");
+  if (one_qubit) {
     for (uint32_t i = 0; i < NR; ++i) {
       if ((i & rc) != rc) continue;
+      if (kind == qs::dev::kOp1Z) {
+        if ((i >> b0) & 1) v[i] = -v[i];
+        continue;
+      }
       if (kind == qs::dev::kOp1Diag) {
         v[i] *= ((i >> b0) & 1) ? m[3] : m[0];
         continue;
@@ -161,7 +166,7 @@ void replay(cplx *psi, unsigned n, const qs::TileProgram &prog, unsigned rb, boo
             if ((base & op.gcmask) != op.gcmask) continue;
             if ((lb & op.lcmask) != op.lcmask) continue;
             const uint32_t kd = (op.code >> 8) & 15u;
-            const uint32_t width = (kd < 4 || kd == qs::dev::kOp1Real) ? 4 : 16;
+            const uint32_t width = (kd < 4 || kd == qs::dev::kOp1Real || kd == qs::dev::kOp1Z) ? 4 : 16;
             check(op.mat + width <= prog.mats.size(), "matrix offset");
             apply_small(v, op.code, op.rcmask, prog.mats.data() + op.mat, rb);
           }
@@ -217,9 +222,20 @@ std::vector<qs::PGate> read_gates(int n_gates, const uint32_t *k, const uint32_t
   return gates;
 }
 
+// The gate list the engine's window planners see: CNOTs rewritten as H, CZ, H where the engine
+// does it (emu_set_cz), then single-qubit runs merged.
+bool g_cz = false;
+std::vector<qs::PGate> plan_gates(int n_gates, const uint32_t *k, const uint32_t *tpos, const uint64_t *cmask,
+                                  const double *mats) {
+  auto g = read_gates(n_gates, k, tpos, cmask, mats);
+  return qs::merge_single_qubit_runs(g_cz ? qs::cnot_to_cz(g) : g);
+}
+
 }  // namespace
 
 extern "C" {
+
+void emu_set_cz(int on) { g_cz = on != 0; }
 
 // The LDS swizzle the kernels use (layout.hpp), for the CPU tests.
 uint32_t emu_slot(uint32_t x, int c64) { return slot(x, c64 != 0); }
@@ -230,7 +246,7 @@ int emu_plan_stats(unsigned n, unsigned t, unsigned lmin, unsigned rb, int c64, 
                    const uint32_t *tpos, const uint64_t *cmask, const double *mats, int *stats,
                    uint32_t *stage_bits, int max_stage_bits, char *err, int err_len) {
   try {
-    auto gates = qs::merge_single_qubit_runs(read_gates(n_gates, k, tpos, cmask, mats));
+    auto gates = plan_gates(n_gates, k, tpos, cmask, mats);
     auto passes = qs::schedule_tiles(gates, qs::TileShape{n, t, lmin});
     int reg = 0, lds = 0, rops = 0;
     for (auto &ps : passes) {
@@ -269,7 +285,7 @@ int emu_run_relabel(unsigned n, unsigned t, unsigned lmin, unsigned rb, int c64,
                     const uint32_t *k, const uint32_t *tpos, const uint64_t *cmask, const double *mats, double *state,
                     int *n_passes, uint32_t *final_pos, char *err, int err_len) {
   try {
-    auto gates = qs::merge_single_qubit_runs(read_gates(n_gates, k, tpos, cmask, mats));
+    auto gates = plan_gates(n_gates, k, tpos, cmask, mats);
     auto passes = qs::schedule_tiles(gates, qs::TileShape{n, t, lmin}, qs::slot_chooser(rule), 256, 16.0, relabel != 0);
     std::vector<uint32_t> label(n), phys(n);
     for (uint32_t p = 0; p < n; ++p) label[p] = phys[p] = p;
@@ -305,7 +321,7 @@ int emu_run_relabel(unsigned n, unsigned t, unsigned lmin, unsigned rb, int c64,
 int emu_run(unsigned n, unsigned t, unsigned lmin, unsigned rb, int c64, int n_gates, const uint32_t *k, const uint32_t *tpos,
             const uint64_t *cmask, const double *mats, double *state, int *n_passes, char *err, int err_len) {
   try {
-    auto gates = qs::merge_single_qubit_runs(read_gates(n_gates, k, tpos, cmask, mats));
+    auto gates = plan_gates(n_gates, k, tpos, cmask, mats);
     auto passes = qs::schedule_tiles(gates, qs::TileShape{n, t, lmin});
     size_t covered = 0;
     if (!state) {  // planning only: count passes
@@ -337,15 +353,15 @@ extern "C" int emu_op_kinds(unsigned n, unsigned t, unsigned lmin, unsigned rb, 
                             const uint32_t *tpos, const uint64_t *cmask, const double *mats, int *counts, char *err,
                             int err_len) {
   try {
-    auto gates = qs::merge_single_qubit_runs(read_gates(n_gates, k, tpos, cmask, mats));
+    auto gates = plan_gates(n_gates, k, tpos, cmask, mats);
     auto passes = qs::schedule_tiles(gates, qs::TileShape{n, t, lmin});
-    for (int i = 0; i < 14; ++i) counts[i] = 0;
+    for (int i = 0; i < 16; ++i) counts[i] = 0;
     for (auto &ps : passes) {
       qs::TileProgram prog;
       qs::build_tile_pass(gates, ps, std::min(t, n), rb, c64 != 0, prog);
       for (const auto &op : prog.rops) {
         const unsigned kind = (op.code >> 8) & 15u;
-        if (kind < 7) counts[(op.lcmask != 0 ? 7 : 0) + kind] += 1;
+        if (kind < 8) counts[(op.lcmask != 0 ? 8 : 0) + kind] += 1;
       }
     }
     return 0;
@@ -361,7 +377,7 @@ extern "C" int emu_max_pass_upload(unsigned n, unsigned t, unsigned lmin, unsign
                                    const uint32_t *k, const uint32_t *tpos, const uint64_t *cmask,
                                    const double *mats, uint64_t *max_bytes, int *n_passes, char *err, int err_len) {
   try {
-    auto gates = qs::merge_single_qubit_runs(read_gates(n_gates, k, tpos, cmask, mats));
+    auto gates = plan_gates(n_gates, k, tpos, cmask, mats);
     auto passes = qs::schedule_tiles(gates, qs::TileShape{n, t, lmin});
     auto up = [](size_t b) { return (b + 255) & ~size_t(255); };
     uint64_t most = 0;

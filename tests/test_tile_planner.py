@@ -174,6 +174,49 @@ def test_relabelling_saves_passes(emu):
             assert relab < 0.9 * plain, (lmin, rule, relab, plain)
 
 
+@pytest.fixture
+def emu_cz(emu):
+    """The emulator planning the gate list the engine's window planners see: CNOTs between
+    1-qubit gates on their target rewritten as H, controlled Z, H (cnot_to_cz)."""
+    emu.emu_set_cz(1)
+    yield emu
+    emu.emu_set_cz(0)
+
+
+@pytest.mark.parametrize("name,make", CASES, ids=[c[0] for c in CASES])
+@pytest.mark.parametrize("n,t,lmin,rb,c64", [(12, 10, 4, 4, True), (14, 10, 5, 5, False), (15, 12, 4, 4, True),
+                                              (16, 13, 5, 4, True)])
+@pytest.mark.parametrize("rule", [0, 1])
+def test_cnots_as_controlled_z_match_unfused(emu_cz, name, make, n, t, lmin, rb, c64, rule):
+    """With CNOTs rewritten as H, CZ, H (the H's merged into the neighbouring 1-qubit gates) the
+    relabelling tile plan still gives the unfused result."""
+    cmds = make(n)
+    want = oracle_state(cmds, n)
+    got, _, _ = run_emu_relabel(emu_cz, n, t, lmin, cmds, start_state(n), rb, c64, True, rule)
+    assert np.max(np.abs(got - want)) <= 1e-12
+
+
+def test_cnots_as_controlled_z_replace_register_exchanges(emu, emu_cz):
+    """On a 24-qubit random-layer circuit most CNOTs sit between 1-qubit gates on their target: the
+    rewrite turns a share of the X ops (register exchanges) into Z ops (sign flips in place), and
+    CNOTs with a neighbour that is no uncontrolled 1-qubit gate on the target stay X ops."""
+    cmds = circuits.random_layers(24, 20, 12, 7, measure=False)
+    emu.emu_set_cz(0)
+    plain0, lane0 = op_kind_counts(emu, 24, 12, 4, 4, cmds, True)
+    emu.emu_set_cz(1)
+    plain1, lane1 = op_kind_counts(emu, 24, 12, 4, 4, cmds, True)
+    x0, x1 = plain0[3] + lane0[3], plain1[3] + lane1[3]
+    z0, z1 = plain0[7] + lane0[7], plain1[7] + lane1[7]
+    assert z0 == 0 and z1 > 0.25 * x0 and x1 + z1 == x0, (x0, x1, z1)
+    # CX(0 -> 1) stays an X op (qubit 1 is next a control); CX(1 -> 2) sits between H and Rz on
+    # qubit 2 and becomes a Z op
+    n = 6
+    chain = circuits.allocate(n) + [commands.gate(G.H, [q]) for q in range(n)] + [
+        commands.gate(G.X, [1], [0]), commands.gate(G.X, [2], [1]), commands.gate(G.Rz(0.3), [2])]
+    p, l = op_kind_counts(emu, n, 6, 4, 4, chain, True)
+    assert p[3] + l[3] == 1 and p[7] + l[7] == 1
+
+
 def test_random_controlled_blocks(emu):
     rng = np.random.default_rng(4)
     n = 13
@@ -282,13 +325,14 @@ def test_register_stages_spread_over_lds_banks(emu):
 
 
 def lane_controlled_mix(n, n_gates, seed):
-    """Controlled gates of every register-op kind (general, real, diagonal, anti-diagonal, X,
+    """Controlled gates of every register-op kind (general, real, diagonal, anti-diagonal, X, Z,
     2-qubit general, 2-qubit diagonal) with 1-2 controls on random qubits, so that many controls
     fall on tile bits outside a stage's register bits."""
     rng = np.random.default_rng(seed)
     makers = [
         lambda a: (G.Rx(a), 1), lambda a: (G.H, 1), lambda a: (G.Ry(a), 1), lambda a: (G.Rz(a), 1), lambda a: (G.R(a), 1),
         lambda a: (G.Y, 1), lambda a: (G.X, 1), lambda a: (G.Rxx(a), 2), lambda a: (G.Rzz(a), 2),
+        lambda a: (G.Z, 1),
     ]
     cmds = circuits.allocate(n) + [commands.gate(G.H, [q]) for q in range(n)]
     for _ in range(n_gates):
@@ -311,7 +355,7 @@ def op_kind_counts(lib, n, t, lmin, rb, cmds, c64):
     tp = np.asarray(tpos, dtype=np.uint32)
     cmask = np.asarray(cm, dtype=np.uint64)
     mflat = np.ascontiguousarray(np.concatenate(mats))
-    counts = np.zeros(14, dtype=np.int32)
+    counts = np.zeros(16, dtype=np.int32)
     err = ctypes.create_string_buffer(256)
     rc = lib.emu_op_kinds(
         ctypes.c_uint(n), ctypes.c_uint(t), ctypes.c_uint(lmin), ctypes.c_uint(rb), ctypes.c_int(int(c64)),
@@ -320,7 +364,7 @@ def op_kind_counts(lib, n, t, lmin, rb, cmds, c64):
         counts.ctypes.data_as(ctypes.c_void_p), err, ctypes.c_int(256),
     )
     assert rc == 0, err.value.decode()
-    return counts[:7], counts[7:]
+    return counts[:8], counts[8:]
 
 
 LANE_MIX = dict(n=15, n_gates=400, seed=21)
@@ -328,7 +372,7 @@ LANE_MIX = dict(n=15, n_gates=400, seed=21)
 
 @pytest.mark.parametrize("c64", [True, False])
 def test_lane_controlled_mix_covers_every_predicated_op_kind(emu, c64):
-    """The circuit of the GPU test below plans (t=12, RB=4, L=4) into register ops of all seven
+    """The circuit of the GPU test below plans (t=12, RB=4, L=4) into register ops of all eight
     kinds both with and without tile-bit controls, so that test runs every plain and every
     predicated op form of the kernel; the CPU replay of the plan matches the unfused oracle."""
     cmds = lane_controlled_mix(**LANE_MIX)
