@@ -250,6 +250,11 @@ void Batch::build(const BatchShape &shape) {
   d_state_off_ = to_device(state_off);
   d_stages_ = to_device(stages);
   d_rops_ = to_device(rops);
+  two_qubit_ops_ = false;
+  for (const dev::RegOp &op : rops) {
+    const uint32_t kind = (op.code >> 8) & 15u;
+    two_qubit_ops_ = two_qubit_ops_ || kind == dev::kOp2General || kind == dev::kOp2Diag;
+  }
   d_tops_ = to_device(tops);
   if (c64_) {
     std::vector<float2> m(mats.size());
@@ -287,7 +292,7 @@ void Batch::enqueue() {
     for (size_t l = 0; l < launch_jobs_.size(); ++l)
       dev::batch_tile_pass<R>(psi, d_jobs_[l], d_block_job_[l], launch_blocks_[l], launch_tmax_[l], (int)shape_.rb,
                               (const dev::TileStage *)d_stages_, (const dev::RegOp *)d_rops_,
-                              (const dev::TileOp *)d_tops_, (const amp_t<R> *)d_mats_, stream_);
+                              (const dev::TileOp *)d_tops_, (const amp_t<R> *)d_mats_, two_qubit_ops_, stream_);
     dev::batch_energy<R>(psi, d_ejobs_, d_eblock_, e_blocks_, d_terms_, d_partials_, stream_);
   };
   if (c64_) go(float{});

@@ -50,6 +50,7 @@ class Batch {
   bool c64_;
   std::vector<BatchCircuit> circ_;
   BatchShape shape_;
+  bool two_qubit_ops_ = true;  // any 2-qubit register op in the batch (picks the tile kernel)
   hipStream_t stream_ = nullptr;
   hipEvent_t ev0_ = nullptr, ev1_ = nullptr;
   // device data

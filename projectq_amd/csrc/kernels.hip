@@ -678,7 +678,7 @@ __device__ __forceinline__ void run_stages(amp_t<R> *__restrict__ tile, uint32_t
 // `seg` holds the 2^h segment offsets of the pass (shared memory, filled by the caller).
 // FULL: 2^t is a multiple of kStreamChunk * kBlock * VN, so every streamed vector exists and the
 // accesses need no guard (straight-line code lets the memory counters be waited on exactly).
-template <typename R, int RB, bool FULL>
+template <typename R, int RB, bool FULL, bool TWO = true>
 __device__ __forceinline__ void tile_body(amp_t<R> *__restrict__ psi, uint64_t tix, const TilePass &P,
                                           const uint64_t *seg, amp_t<R> *__restrict__ tile,
                                           const TileStage *__restrict__ stages,
@@ -705,7 +705,7 @@ __device__ __forceinline__ void tile_body(amp_t<R> *__restrict__ psi, uint64_t t
     }
   }
   __syncthreads();
-  run_stages<R, RB, 1>(tile, t, base, P.n_stages, stages, rops, tops, mats);
+  run_stages<R, RB, 1, kBlock, TWO>(tile, t, base, P.n_stages, stages, rops, tops, mats);
   // the store reads the computed tile through the relabelling (linear: sources of disjoint bit
   // sets combine by XOR)
   const uint32_t src_thread = store_source(threadIdx.x * VN, P), pair = slot_bytes<R>(store_source(1, P));
@@ -737,7 +737,8 @@ __device__ __forceinline__ void fill_segments(const TilePass &P, uint64_t *seg) 
   __syncthreads();
 }
 
-template <typename R, int RB, bool FULL>
+// TWO = false: the instantiation for passes without 2-qubit register ops (see reg_op)
+template <typename R, int RB, bool FULL, bool TWO = true>
 __global__ __launch_bounds__(kBlock) void k_tile_pass(amp_t<R> *__restrict__ psi, uint64_t n_tiles,
                                                       TilePass P, const TileStage *__restrict__ stages,
                                                       const RegOp *__restrict__ rops,
@@ -750,7 +751,7 @@ __global__ __launch_bounds__(kBlock) void k_tile_pass(amp_t<R> *__restrict__ psi
   uint64_t *seg = reinterpret_cast<uint64_t *>(smem + (sizeof(amp_t<R>) << P.t));
   fill_segments(P, seg);
   for (uint64_t tix = blockIdx.x; tix < n_tiles; tix += gridDim.x)
-    tile_body<R, RB, FULL>(psi, tix, P, seg, tile, stages, rops, tops, mats);
+    tile_body<R, RB, FULL, TWO>(psi, tix, P, seg, tile, stages, rops, tops, mats);
 }
 
 // Software-pipelined tile pass: while the stages of tile i run out of LDS, the loads of the
@@ -839,7 +840,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_
 
 // Many circuits at once: workgroup b runs tile (b - job.first_block) of job block_job[b]; every
 // job is one pass of one circuit whose state starts at psi + job.state_off.
-template <typename R, int RB>
+template <typename R, int RB, bool TWO = true>
 __global__ __launch_bounds__(kBlock) void k_batch_tile_pass(amp_t<R> *__restrict__ psi,
                                                             const BatchJob *__restrict__ jobs,
                                                             const uint32_t *__restrict__ block_job,
@@ -853,7 +854,7 @@ __global__ __launch_bounds__(kBlock) void k_batch_tile_pass(amp_t<R> *__restrict
   amp_t<R> *tile = reinterpret_cast<amp_t<R> *>(smem);
   uint64_t *seg = reinterpret_cast<uint64_t *>(smem + (sizeof(amp_t<R>) << P.t));
   fill_segments(P, seg);
-  tile_body<R, RB, false>(psi + job->state_off, blockIdx.x - job->first_block, P, seg, tile, stages + job->stage_off,
+  tile_body<R, RB, false, TWO>(psi + job->state_off, blockIdx.x - job->first_block, P, seg, tile, stages + job->stage_off,
                    rops + job->rop_off, tops + job->top_off, mats + job->mat_off);
 }
 
@@ -1511,6 +1512,13 @@ void launch_tile_pass(amp_t<R> *psi, uint64_t n_tiles, const TilePass &P, const 
       done = true;
     }
   }
+  if constexpr (RB == 4) {  // passes without 2-qubit register ops: the kernel without those forms
+    if (!done && tsize % chunk == 0 && !two_qubit_ops) {
+      hipLaunchKernelGGL((k_tile_pass<R, RB, true, false>), dim3((unsigned)grid), dim3(kBlock), lds, s, psi,
+                         n_tiles, P, d_stages, d_rops, d_tops, d_mats);
+      done = true;
+    }
+  }
   if (done) {
   } else if (tsize % chunk == 0)
     hipLaunchKernelGGL((k_tile_pass<R, RB, true>), dim3((unsigned)grid), dim3(kBlock), lds, s, psi, n_tiles, P,
@@ -1542,6 +1550,8 @@ void prepare_tile_kernels() {
   QS_HIP(hipFuncSetAttribute((const void *)k_tile_pass<float, 5, true>, hipFuncAttributeMaxDynamicSharedMemorySize, kTileLdsMax<float>));
   QS_HIP(hipFuncSetAttribute((const void *)k_tile_pass<double, 4, true>, hipFuncAttributeMaxDynamicSharedMemorySize, kTileLdsMax<double>));
   QS_HIP(hipFuncSetAttribute((const void *)k_tile_pass<double, 5, true>, hipFuncAttributeMaxDynamicSharedMemorySize, kTileLdsMax<double>));
+  QS_HIP(hipFuncSetAttribute((const void *)k_tile_pass<float, 4, true, false>, hipFuncAttributeMaxDynamicSharedMemorySize, kTileLdsMax<float>));
+  QS_HIP(hipFuncSetAttribute((const void *)k_tile_pass<double, 4, true, false>, hipFuncAttributeMaxDynamicSharedMemorySize, kTileLdsMax<double>));
   QS_HIP(hipFuncSetAttribute((const void *)k_tile_pass<float, 4, false>, hipFuncAttributeMaxDynamicSharedMemorySize, kTileLdsMax<float>));
   QS_HIP(hipFuncSetAttribute((const void *)k_tile_pass<float, 5, false>, hipFuncAttributeMaxDynamicSharedMemorySize, kTileLdsMax<float>));
   QS_HIP(hipFuncSetAttribute((const void *)k_tile_pass<double, 4, false>, hipFuncAttributeMaxDynamicSharedMemorySize, kTileLdsMax<double>));
@@ -1554,6 +1564,8 @@ void prepare_tile_kernels() {
   QS_HIP(hipFuncSetAttribute((const void *)k_batch_tile_pass<float, 5>, hipFuncAttributeMaxDynamicSharedMemorySize, kTileLdsMax<float>));
   QS_HIP(hipFuncSetAttribute((const void *)k_batch_tile_pass<double, 4>, hipFuncAttributeMaxDynamicSharedMemorySize, kTileLdsMax<double>));
   QS_HIP(hipFuncSetAttribute((const void *)k_batch_tile_pass<double, 5>, hipFuncAttributeMaxDynamicSharedMemorySize, kTileLdsMax<double>));
+  QS_HIP(hipFuncSetAttribute((const void *)k_batch_tile_pass<float, 4, false>, hipFuncAttributeMaxDynamicSharedMemorySize, kTileLdsMax<float>));
+  QS_HIP(hipFuncSetAttribute((const void *)k_batch_tile_pass<double, 4, false>, hipFuncAttributeMaxDynamicSharedMemorySize, kTileLdsMax<double>));
   done = true;
 }
 
@@ -1568,20 +1580,32 @@ void batch_init(amp_t<R> *psi, uint64_t total, const uint64_t *state_off, int n_
 template <typename R, int RB>
 void launch_batch_tile_pass(amp_t<R> *psi, const BatchJob *jobs, const uint32_t *block_job, uint32_t n_blocks,
                             uint32_t max_t, const TileStage *stages, const RegOp *rops, const TileOp *tops,
-                            const amp_t<R> *mats, hipStream_t s) {
+                            const amp_t<R> *mats, bool two_qubit_ops, hipStream_t s) {
   prepare_tile_kernels();
-  hipLaunchKernelGGL((k_batch_tile_pass<R, RB>), dim3(n_blocks), dim3(kBlock), (sizeof(amp_t<R>) << max_t) + (sizeof(uint64_t) << kTileMaxHigh), s, psi,
-                     jobs, block_job, stages, rops, tops, mats);
+  const size_t lds = (sizeof(amp_t<R>) << max_t) + (sizeof(uint64_t) << kTileMaxHigh);
+  bool done = false;
+  if constexpr (RB == 4) {
+    if (!two_qubit_ops) {
+      hipLaunchKernelGGL((k_batch_tile_pass<R, RB, false>), dim3(n_blocks), dim3(kBlock), lds, s, psi, jobs,
+                         block_job, stages, rops, tops, mats);
+      done = true;
+    }
+  }
+  if (!done)
+    hipLaunchKernelGGL((k_batch_tile_pass<R, RB>), dim3(n_blocks), dim3(kBlock), lds, s, psi, jobs, block_job, stages,
+                       rops, tops, mats);
   QS_HIP(hipGetLastError());
 }
 
 template <typename R>
 void batch_tile_pass(amp_t<R> *psi, const BatchJob *jobs, const uint32_t *block_job, uint32_t n_blocks,
                      uint32_t max_t, int rb, const TileStage *stages, const RegOp *rops, const TileOp *tops,
-                     const amp_t<R> *mats, hipStream_t s) {
+                     const amp_t<R> *mats, bool two_qubit_ops, hipStream_t s) {
   if (max_t > (uint32_t)kTileMaxBits || n_blocks == 0) throw ValueError("batch pass: bad launch shape");
-  if (rb == 5) launch_batch_tile_pass<R, 5>(psi, jobs, block_job, n_blocks, max_t, stages, rops, tops, mats, s);
-  else if (rb == 4) launch_batch_tile_pass<R, 4>(psi, jobs, block_job, n_blocks, max_t, stages, rops, tops, mats, s);
+  if (rb == 5)
+    launch_batch_tile_pass<R, 5>(psi, jobs, block_job, n_blocks, max_t, stages, rops, tops, mats, two_qubit_ops, s);
+  else if (rb == 4)
+    launch_batch_tile_pass<R, 4>(psi, jobs, block_job, n_blocks, max_t, stages, rops, tops, mats, two_qubit_ops, s);
   else throw ValueError("batch pass: register bits must be 4 or 5");
 }
 
@@ -1775,7 +1799,7 @@ template void convert_amps<float, double>(const amp_t<float> *, amp_t<double> *,
   template void batch_init<R>(amp_t<R> *, uint64_t, const uint64_t *, int, hipStream_t);          \
   template void batch_tile_pass<R>(amp_t<R> *, const BatchJob *, const uint32_t *, uint32_t,       \
                                    uint32_t, int, const TileStage *, const RegOp *, const TileOp *,\
-                                   const amp_t<R> *, hipStream_t);                                \
+                                   const amp_t<R> *, bool, hipStream_t);                                \
   template void tile_pass<R>(amp_t<R> *, uint64_t, const TilePass &, int, const TileStage *,      \
                              const RegOp *, const TileOp *, const amp_t<R> *, bool, hipStream_t); \
   template void apply_block<R>(amp_t<R> *, uint64_t, const BitInsert &, uint64_t,                 \

@@ -52,11 +52,12 @@ void prepare_tile_kernels();
 // the sequence can be captured in a graph)
 template <typename R>
 void batch_init(amp_t<R> *psi, uint64_t total_amps, const uint64_t *state_off, int n_states, hipStream_t s);
-// one pass of every circuit: n_blocks workgroups, block_job maps each to its job
+// one pass of every circuit: n_blocks workgroups, block_job maps each to its job;
+// two_qubit_ops: whether any register op of the batch is a 2-qubit one
 template <typename R>
 void batch_tile_pass(amp_t<R> *psi, const BatchJob *jobs, const uint32_t *block_job, uint32_t n_blocks,
                      uint32_t max_t, int rb, const TileStage *stages, const RegOp *rops, const TileOp *tops,
-                     const amp_t<R> *mats, hipStream_t s);
+                     const amp_t<R> *mats, bool two_qubit_ops, hipStream_t s);
 // partials[b] for every energy workgroup, then energies[j] = sum of job j's partials in order
 template <typename R>
 void batch_energy(const amp_t<R> *psi, const EnergyJob *jobs, const EnergyBlock *blocks, uint32_t n_blocks,
