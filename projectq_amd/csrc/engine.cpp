@@ -518,7 +518,11 @@ std::vector<PGate> Engine::resolve(const std::vector<Gate> &gates) const {
     if (p.tmask() & p.cmask) continue;  // a control that is also a target: never applies
     pg.push_back(std::move(p));
   }
-  return merge_single_qubit_runs(cnot_cz_ ? cnot_to_cz(pg) : pg);
+  // CNOTs become H, CZ, H only where every pass will run the 1-qubit kernel, whose sign flip
+  // makes the CZ cheap: in windows without 2-qubit gates (the only source of 2-qubit register ops)
+  bool two_qubit_gates = false;
+  for (const PGate &p : pg) two_qubit_gates = two_qubit_gates || p.tpos.size() == 2;
+  return merge_single_qubit_runs(cnot_cz_ && !two_qubit_gates ? cnot_to_cz(pg) : pg);
 }
 
 // Tile size: 2^t amplitudes of LDS per workgroup, at most n bits, and small enough that a state

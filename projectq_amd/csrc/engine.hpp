@@ -168,8 +168,8 @@ class Engine {
   // qubit relabelling at tile stores (schedule_tiles): -1 the default (on for the learned
   // planner, off for QS_PLANNER_TILE), 0 off, 1 on; PROJECTQ_AMD_RELABEL=0|1 sets it
   int relabel_mode_ = -1;
-  // window planners: CNOTs between 1-qubit gates on their target become H, CZ, H (cnot_to_cz);
-  // PROJECTQ_AMD_CZ=0 keeps them as X ops
+  // window planners: CNOTs between 1-qubit gates on their target become H, CZ, H (cnot_to_cz) in
+  // windows without 2-qubit gates; PROJECTQ_AMD_CZ=0 keeps them as X ops
   bool cnot_cz_ = true;
   void apply_relabel(const std::vector<std::pair<uint32_t, uint32_t>> &swaps);
   void exec_tile_program(const TileProgram &prog, unsigned rb);
