@@ -39,33 +39,43 @@ std::vector<PGate> merge_single_qubit_runs(const std::vector<PGate> &gates) {
 
 std::vector<PGate> cnot_to_cz(const std::vector<PGate> &gates) {
   const std::complex<double> zero(0.0, 0.0), one(1.0, 0.0);
-  auto uncontrolled_single_on = [&](int i, uint32_t p) {
-    return i >= 0 && i < (int)gates.size() && gates[i].tpos.size() == 1 && gates[i].tpos[0] == p &&
-           gates[i].cmask == 0;
-  };
-  // previous and next gate touching each gate's target position
   const int n = (int)gates.size();
-  std::vector<int> prev(n, -1), next(n, n);
-  {
-    std::map<uint32_t, int> last;
-    for (int i = 0; i < n; ++i) {
-      const uint64_t touch = gates[i].tmask() | gates[i].cmask;
-      if (gates[i].tpos.size() == 1) {
-        auto it = last.find(gates[i].tpos[0]);
-        if (it != last.end()) prev[i] = it->second;
+  auto is_cnot = [&](int i) {
+    const PGate &g = gates[i];
+    return g.tpos.size() == 1 && g.cmask != 0 && g.m.size() == 4 && g.m[0] == zero && g.m[1] == one &&
+           g.m[2] == one && g.m[3] == zero;
+  };
+  auto single_on = [&](int i, uint32_t p) {
+    return i >= 0 && i < n && gates[i].tpos.size() == 1 && gates[i].tpos[0] == p && gates[i].cmask == 0;
+  };
+  // gates touching each position, in order
+  std::map<uint32_t, std::vector<int>> seq;
+  for (int i = 0; i < n; ++i) {
+    const uint64_t touch = gates[i].tmask() | gates[i].cmask;
+    for (uint32_t p = 0; p < 64; ++p)
+      if ((touch >> p) & 1) seq[p].push_back(i);
+  }
+  // runs of CNOTs on one target with nothing else touching it in between: X_1 .. X_k = H Z_1 .. Z_k H,
+  // so a run between uncontrolled 1-qubit gates on the target takes one H before its first CNOT
+  // and one after its last
+  std::vector<char> h_before(n, 0), h_after(n, 0), as_cz(n, 0);
+  for (auto &kv : seq) {
+    const uint32_t t = kv.first;
+    const std::vector<int> &s = kv.second;
+    for (size_t a = 0; a < s.size();) {
+      if (!is_cnot(s[a]) || gates[s[a]].tpos[0] != t) {
+        ++a;
+        continue;
       }
-      for (uint32_t p = 0; p < 64; ++p)
-        if ((touch >> p) & 1) last[p] = i;
-    }
-    last.clear();
-    for (int i = n - 1; i >= 0; --i) {
-      const uint64_t touch = gates[i].tmask() | gates[i].cmask;
-      if (gates[i].tpos.size() == 1) {
-        auto it = last.find(gates[i].tpos[0]);
-        if (it != last.end()) next[i] = it->second;
+      size_t b = a;
+      while (b + 1 < s.size() && is_cnot(s[b + 1]) && gates[s[b + 1]].tpos[0] == t) ++b;
+      const int before = a > 0 ? s[a - 1] : -1, after = b + 1 < s.size() ? s[b + 1] : -1;
+      if (single_on(before, t) && single_on(after, t)) {
+        h_before[s[a]] = 1;
+        h_after[s[b]] = 1;
+        for (size_t k = a; k <= b; ++k) as_cz[s[k]] = 1;
       }
-      for (uint32_t p = 0; p < 64; ++p)
-        if ((touch >> p) & 1) last[p] = i;
+      a = b + 1;
     }
   }
   const double h = std::sqrt(0.5);
@@ -74,20 +84,18 @@ std::vector<PGate> cnot_to_cz(const std::vector<PGate> &gates) {
   out.reserve(gates.size() + gates.size() / 2);
   for (int i = 0; i < n; ++i) {
     const PGate &g = gates[i];
-    const bool cnot = g.tpos.size() == 1 && g.cmask != 0 && g.m.size() == 4 && g.m[0] == zero && g.m[1] == one &&
-                      g.m[2] == one && g.m[3] == zero;
-    if (cnot && uncontrolled_single_on(prev[i], g.tpos[0]) && uncontrolled_single_on(next[i], g.tpos[0])) {
-      PGate a, z, b;
-      a.tpos = b.tpos = z.tpos = g.tpos;
-      a.m = b.m = H;
-      z.m = Z;
-      z.cmask = g.cmask;
-      out.push_back(std::move(a));
-      out.push_back(std::move(z));
-      out.push_back(std::move(b));
-    } else {
+    if (!as_cz[i]) {
       out.push_back(g);
+      continue;
     }
+    PGate hg, z;
+    hg.tpos = z.tpos = g.tpos;
+    hg.m = H;
+    z.m = Z;
+    z.cmask = g.cmask;
+    if (h_before[i]) out.push_back(hg);
+    out.push_back(std::move(z));
+    if (h_after[i]) out.push_back(hg);
   }
   return out;
 }

@@ -60,8 +60,9 @@ struct SlotCandidate {
 using SlotChooser =
     std::function<std::vector<uint32_t>(const std::vector<SlotCandidate> &cands, unsigned slots)>;
 
-// A CNOT (controlled X, any controls) whose target's previous and next gates are uncontrolled
-// 1-qubit gates on that target becomes H, controlled Z, H: the same unitary, and after
+// A run of CNOTs (controlled X, any controls) on one target, with nothing else touching the target
+// in between, whose previous and next gates on the target are uncontrolled 1-qubit gates,
+// becomes H, controlled Z, ..., controlled Z, H: the same unitary, and after
 // merge_single_qubit_runs the H's are part of those neighbours.  The tile kernels apply the
 // controlled Z as a sign flip in place (kOp1Z) instead of exchanging registers, and the
 // scheduler may move a diagonal gate past gates that only read its target.

@@ -215,6 +215,16 @@ def test_cnots_as_controlled_z_replace_register_exchanges(emu, emu_cz):
         commands.gate(G.X, [1], [0]), commands.gate(G.X, [2], [1]), commands.gate(G.Rz(0.3), [2])]
     p, l = op_kind_counts(emu, n, 6, 4, 4, chain, True)
     assert p[3] + l[3] == 1 and p[7] + l[7] == 1
+    # two CNOTs onto qubit 2 with nothing else on it in between, between H and Rz: both become Z
+    # ops under one pair of H's
+    run = circuits.allocate(n) + [commands.gate(G.H, [q]) for q in range(n)] + [
+        commands.gate(G.X, [2], [0]), commands.gate(G.Ry(0.2), [4]), commands.gate(G.X, [2], [1]),
+        commands.gate(G.Rz(0.3), [2])]
+    p, l = op_kind_counts(emu, n, 6, 4, 4, run, True)
+    assert p[3] + l[3] == 0 and p[7] + l[7] == 2
+    want = oracle_state(run, n)
+    got, _ = run_emu(emu, n, 6, 4, run, start_state(n), 4, True)
+    assert np.max(np.abs(got - want)) <= 1e-12
 
 
 def test_random_controlled_blocks(emu):
