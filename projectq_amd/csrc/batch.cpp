@@ -23,7 +23,7 @@ T *to_device(const std::vector<T> &v) {
 std::vector<BatchShape> candidate_shapes(bool c64) {
   if (c64)
     return {{12, 4, 5, false}, {11, 4, 5, false}, {10, 4, 4, false}, {12, 5, 5, false},
-            {10, 5, 5, false}, {12, 4, 4, true}, {11, 4, 4, true}, {10, 4, 4, true}};
+            {10, 5, 5, false}, {12, 4, 4, true}, {11, 4, 4, true}, {10, 4, 4, true}, {12, 4, 5, true}};
   return {{12, 4, 4, false}, {11, 4, 4, false}, {10, 4, 4, false}, {12, 4, 4, true}};
 }
 

@@ -79,7 +79,7 @@ def test_batch_shapes_agree():
     # every candidate on its own, including the shapes whose tile stores relabel qubits (the
     # Pauli strings then act on each circuit's final layout)
     relabelled = 0
-    for k in range(8):
+    for k in range(9):
         os.environ["PROJECTQ_AMD_BATCH_SHAPE"] = str(k)
         try:
             for prec, tol in (("c128", 1e-12), ("c64", 2e-5)):
